@@ -1,0 +1,370 @@
+"""Python binding of libfastcon (ctypes over the C ABI in include/fastcon.h).
+
+This is the host-side mirror used by the tests and the benchmark. It loads
+the in-tree ``fast-con_amd/libfastcon.so`` and raises ``ImportError`` when the
+library has not been built: there is no fallback implementation.
+
+Reference interfaces mirrored (paths relative to the reference checkout):
+  Graph.bfs            cudaGraph::Reset + cudaBFS4K (BFS-4K/src/device/BFS_WorkEfficient.cu:81-133)
+  Graph.stcon          make_stcon (bernaschi/ST_CONN_ONE_ATOMIC/current-st/st-conn/make_stcon.c:730-1497)
+  read_graph           readGraph::readGraphHeader + readSTD (BFS-4K/Util/readGraph.h:17-23)
+  csr_from_edges       run_make_struct (st-conn/make_struct.c:304-356,553-738)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfastcon.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C fast-con_amd)")
+_lib = C.CDLL(LIB_PATH)
+
+FC_INF = -1
+OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_IO, ERR_FORMAT, ERR_CAPACITY, ERR_NOGPU, ERR_TIMEOUT, ERR_UNSUPPORTED = range(10)
+POLICY_PLAIN, POLICY_HEURISTIC, POLICY_LEARNED = 0, 1, 2
+DIR_FILE, DIR_DIRECTED, DIR_UNDIRECTED = 0, 1, 2
+SCHED_FEATURES, SCHED_HIDDEN, SCHED_ACTIONS = 12, 32, 6
+SCHED_PARAMS = 12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_f32p = C.POINTER(C.c_float)
+
+
+class FastconError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"fastcon status {status}: {msg}")
+        self.status = status
+
+
+class _HostGraph(C.Structure):
+    _fields_ = [("V", C.c_int64), ("E", C.c_int64), ("offsets", _i64p), ("columns", _i32p),
+                ("directed", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("time_ms", C.c_double), ("levels", C.c_int32), ("depth", C.c_int32),
+                ("visited_vertices", C.c_int64), ("visited_edges", C.c_int64),
+                ("expanded_vertices", C.c_int64), ("expanded_edges", C.c_int64),
+                ("topdown_levels", C.c_int32), ("bottomup_levels", C.c_int32),
+                ("max_frontier", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("policy", C.c_int32), ("vw", C.c_int32), ("tile", C.c_int32), ("use_graph", C.c_int32),
+                ("words", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("time_ms", C.c_double), ("waves", C.c_int32), ("levels", C.c_int32),
+                ("active_vertices", C.c_int64), ("active_edges", C.c_int64), ("touched_vertices", C.c_int64),
+                ("vw", C.c_int32), ("tile", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TraceRow(C.Structure):
+    _fields_ = [("features", C.c_float * SCHED_FEATURES), ("cost_us", C.c_float * SCHED_ACTIONS)]
+
+
+def _sig(name, res, *args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_lib.fc_last_error.restype = C.c_char_p
+_lib.fc_version.restype = C.c_char_p
+_sig("fc_host_graph_free", None, C.POINTER(_HostGraph))
+_sig("fc_free", None, C.c_void_p)
+_sig("fc_read_graph", C.c_int, C.c_char_p, C.c_int, C.POINTER(_HostGraph))
+_sig("fc_read_edge_list", C.c_int, C.c_char_p, C.POINTER(_i64p), _i64p, _i64p)
+_sig("fc_csr_from_edges", C.c_int, _i64p, C.c_int64, C.c_int64, C.POINTER(_HostGraph))
+_sig("fc_gen_rmat", C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
+     C.POINTER(_HostGraph))
+_sig("fc_gen_grid", C.c_int, C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.POINTER(_HostGraph))
+_sig("fc_gen_er", C.c_int, C.c_int64, C.c_double, C.c_uint64, C.POINTER(_HostGraph))
+_sig("fc_sample_roots", C.c_int, C.POINTER(_HostGraph), C.c_int32, C.c_uint64, _i32p)
+_sig("fc_components", C.c_int, C.POINTER(_HostGraph), _i32p)
+_sig("fc_graph_create_csr32", C.c_int, _i32p, _i32p, C.c_int32, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p))
+_sig("fc_graph_create_csr64", C.c_int, _i64p, _i32p, C.c_int64, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p))
+_sig("fc_graph_destroy", C.c_int, C.c_void_p)
+_sig("fc_bfs", C.c_int, C.c_void_p, C.c_int32, _i32p, _i32p, C.POINTER(Stats))
+_sig("fc_graph_levels", C.c_int, C.c_void_p, _i32p)
+_sig("fc_graph_parents", C.c_int, C.c_void_p, _i32p)
+_sig("fc_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.POINTER(Stats))
+_sig("fc_stcon_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(Stats))
+_sig("fc_stcon_sched_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
+     C.POINTER(Stats))
+_sig("fc_stcon_batch", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(BatchOpts),
+     C.POINTER(BatchStats))
+_sig("fc_sched_trace", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.POINTER(TraceRow), C.c_int32, _i32p)
+_sig("fc_sched_train", C.c_int, C.c_void_p, C.POINTER(TraceRow), C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+     _f32p)
+_sig("fc_sched_set_params", C.c_int, C.c_void_p, _f32p, C.c_int32)
+_sig("fc_sched_get_params", C.c_int, C.c_void_p, _f32p, C.c_int32)
+_sig("fc_sched_forward", C.c_int, C.c_void_p, _f32p, C.c_int32, _f32p)
+_sig("fc_sched_gradient", C.c_int, C.c_void_p, _f32p, _f32p, C.c_int32, _f32p, _f32p)
+
+# Every symbol the C header declares (checked by the CPU tests).
+EXPORTED = [
+    "fc_last_error", "fc_version", "fc_host_graph_free", "fc_read_graph", "fc_read_edge_list", "fc_free",
+    "fc_csr_from_edges", "fc_gen_rmat", "fc_gen_grid", "fc_gen_er", "fc_sample_roots", "fc_components",
+    "fc_graph_create_csr32", "fc_graph_create_csr64", "fc_graph_destroy", "fc_bfs", "fc_graph_levels",
+    "fc_graph_parents", "fc_stcon", "fc_stcon_many", "fc_stcon_sched_many", "fc_stcon_batch",
+    "fc_sched_trace", "fc_sched_train", "fc_sched_set_params", "fc_sched_get_params", "fc_sched_forward",
+    "fc_sched_gradient",
+]
+
+
+def lib():
+    return _lib
+
+
+def version() -> str:
+    return _lib.fc_version().decode()
+
+
+def _check(st: int):
+    if st != OK:
+        raise FastconError(st, _lib.fc_last_error().decode(errors="replace"))
+
+
+def _p32(a):
+    return a.ctypes.data_as(_i32p)
+
+
+def _p64(a):
+    return a.ctypes.data_as(_i64p)
+
+
+class HostGraph:
+    """A CSR graph in host memory (int64 offsets, int32 columns)."""
+
+    def __init__(self, offsets: np.ndarray, columns: np.ndarray, directed: bool = False):
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.columns = np.ascontiguousarray(columns, dtype=np.int32)
+        self.directed = bool(directed)
+
+    @property
+    def V(self) -> int:
+        return len(self.offsets) - 1
+
+    @property
+    def E(self) -> int:
+        return int(self.offsets[-1])
+
+    def degree(self) -> np.ndarray:
+        return np.diff(self.offsets)
+
+    @classmethod
+    def _take(cls, hg: _HostGraph) -> "HostGraph":
+        V, E = hg.V, hg.E
+        off = np.ctypeslib.as_array(hg.offsets, shape=(V + 1,)).copy()
+        col = np.ctypeslib.as_array(hg.columns, shape=(max(E, 1),))[:E].copy()
+        d = bool(hg.directed)
+        _lib.fc_host_graph_free(C.byref(hg))
+        return cls(off, col, d)
+
+    def _struct(self) -> _HostGraph:
+        return _HostGraph(self.V, self.E, _p64(self.offsets), _p32(self.columns), int(self.directed), 0)
+
+    def sample_roots(self, n: int, seed: int) -> np.ndarray:
+        out = np.empty(n, np.int32)
+        hg = self._struct()
+        _check(_lib.fc_sample_roots(C.byref(hg), n, seed, _p32(out)))
+        return out
+
+    def components(self) -> np.ndarray:
+        out = np.empty(self.V, np.int32)
+        hg = self._struct()
+        _check(_lib.fc_components(C.byref(hg), _p32(out)))
+        return out
+
+
+def gen_rmat(scale: int, edgefactor: float = 16.0, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1) -> HostGraph:
+    hg = _HostGraph()
+    _check(_lib.fc_gen_rmat(scale, edgefactor, a, b, c, seed, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def gen_grid(rows: int, cols: int, p_delete: float = 0.0, seed: int = 1) -> HostGraph:
+    hg = _HostGraph()
+    _check(_lib.fc_gen_grid(rows, cols, p_delete, seed, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def gen_er(n: int, mean_degree: float, seed: int = 1) -> HostGraph:
+    hg = _HostGraph()
+    _check(_lib.fc_gen_er(n, mean_degree, seed, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def csr_from_edges(edges: np.ndarray, V: int = 0) -> HostGraph:
+    e = np.ascontiguousarray(edges, dtype=np.int64).reshape(-1)
+    hg = _HostGraph()
+    _check(_lib.fc_csr_from_edges(_p64(e), len(e) // 2, V, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def read_graph(path: str, direction: int = DIR_FILE) -> HostGraph:
+    hg = _HostGraph()
+    _check(_lib.fc_read_graph(path.encode(), direction, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def read_edge_list(path: str):
+    p = _i64p()
+    m = C.c_int64()
+    nv = C.c_int64()
+    _check(_lib.fc_read_edge_list(path.encode(), C.byref(p), C.byref(m), C.byref(nv)))
+    arr = np.ctypeslib.as_array(p, shape=(max(2 * m.value, 1),))[: 2 * m.value].copy()
+    _lib.fc_free(C.cast(p, C.c_void_p))
+    return arr.reshape(-1, 2), nv.value
+
+
+@dataclass
+class StResult:
+    connected: np.ndarray
+    dist: np.ndarray
+    stats: dict
+
+
+class Graph:
+    """A graph resident on the GPU (library-owned device copy + workspace)."""
+
+    def __init__(self, hg: HostGraph, force64: bool = False):
+        self.V, self.E = hg.V, hg.E
+        self._h = C.c_void_p()
+        if hg.E < 2**31 - 1 and not force64:
+            off32 = hg.offsets.astype(np.int32)
+            _check(_lib.fc_graph_create_csr32(_p32(off32), _p32(hg.columns), hg.V, hg.E, 0, C.byref(self._h)))
+            self.offsets64 = False
+        else:
+            _check(_lib.fc_graph_create_csr64(_p64(hg.offsets), _p32(hg.columns), hg.V, hg.E, 0,
+                                              C.byref(self._h)))
+            self.offsets64 = True
+
+    def close(self):
+        if self._h:
+            _lib.fc_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def bfs(self, src: int, levels: bool = True, parents: bool = False):
+        lv = np.empty(self.V, np.int32) if levels else None
+        pa = np.empty(self.V, np.int32) if parents else None
+        st = Stats()
+        _check(_lib.fc_bfs(self._h, src, _p32(lv) if lv is not None else None,
+                           _p32(pa) if pa is not None else None, C.byref(st)))
+        return lv, pa, st.as_dict()
+
+    def levels(self) -> np.ndarray:
+        lv = np.empty(self.V, np.int32)
+        _check(_lib.fc_graph_levels(self._h, _p32(lv)))
+        return lv
+
+    def stcon(self, s: int, t: int):
+        conn, dist, meet = C.c_int32(), C.c_int32(), C.c_int32()
+        st = Stats()
+        _check(_lib.fc_stcon(self._h, s, t, C.byref(conn), C.byref(dist), C.byref(meet), C.byref(st)))
+        return bool(conn.value), dist.value, meet.value, st.as_dict()
+
+    def stcon_many(self, s, t, policy: int = POLICY_PLAIN) -> StResult:
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        conn = np.empty(len(s), np.int32)
+        dist = np.empty(len(s), np.int32)
+        st = Stats()
+        if policy == POLICY_PLAIN:
+            _check(_lib.fc_stcon_many(self._h, _p32(s), _p32(t), len(s), _p32(conn), _p32(dist), C.byref(st)))
+        else:
+            _check(_lib.fc_stcon_sched_many(self._h, _p32(s), _p32(t), len(s), policy, _p32(conn), _p32(dist),
+                                            C.byref(st)))
+        return StResult(conn, dist, st.as_dict())
+
+    def stcon_batch(self, s, t, policy: int = POLICY_HEURISTIC, vw: int = 0, tile: int = 0, use_graph: int = 1,
+                    words: int = 1) -> StResult:
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        conn = np.empty(len(s), np.int32)
+        dist = np.empty(len(s), np.int32)
+        o = BatchOpts(policy, vw, tile, use_graph, words)
+        st = BatchStats()
+        _check(_lib.fc_stcon_batch(self._h, _p32(s), _p32(t), len(s), _p32(conn), _p32(dist), C.byref(o),
+                                   C.byref(st)))
+        return StResult(conn, dist, st.as_dict())
+
+    # ---- learned scheduler ----
+    def sched_trace(self, s, t, cap: int = 4096) -> np.ndarray:
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        rows = (TraceRow * cap)()
+        n = C.c_int32()
+        _check(_lib.fc_sched_trace(self._h, _p32(s), _p32(t), len(s), rows, cap, C.byref(n)))
+        out = np.zeros((n.value, SCHED_FEATURES + SCHED_ACTIONS), np.float32)
+        for i in range(n.value):
+            out[i, :SCHED_FEATURES] = rows[i].features[:]
+            out[i, SCHED_FEATURES:] = rows[i].cost_us[:]
+        return out
+
+    @staticmethod
+    def _rows(table: np.ndarray):
+        rows = (TraceRow * len(table))()
+        for i, r in enumerate(table):
+            rows[i].features[:] = [float(x) for x in r[:SCHED_FEATURES]]
+            rows[i].cost_us[:] = [float(x) for x in r[SCHED_FEATURES:]]
+        return rows
+
+    def sched_train(self, table: np.ndarray, epochs: int = 200, lr: float = 1e-2, seed: int = 1) -> float:
+        rows = self._rows(table)
+        loss = C.c_float()
+        _check(_lib.fc_sched_train(self._h, rows, len(table), epochs, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def sched_set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(_lib.fc_sched_set_params(self._h, w.ctypes.data_as(_f32p), len(w)))
+
+    def sched_get_params(self) -> np.ndarray:
+        w = np.empty(SCHED_PARAMS, np.float32)
+        _check(_lib.fc_sched_get_params(self._h, w.ctypes.data_as(_f32p), len(w)))
+        return w
+
+    def sched_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((len(x), SCHED_ACTIONS), np.float32)
+        _check(_lib.fc_sched_forward(self._h, x.ctypes.data_as(_f32p), len(x), y.ctypes.data_as(_f32p)))
+        return y
+
+    def sched_gradient(self, x: np.ndarray, target: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        tg = np.ascontiguousarray(target, np.float32)
+        g = np.empty(SCHED_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(_lib.fc_sched_gradient(self._h, x.ctypes.data_as(_f32p), tg.ctypes.data_as(_f32p), len(x),
+                                      C.byref(loss), g.ctypes.data_as(_f32p)))
+        return loss.value, g

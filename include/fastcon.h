@@ -1,0 +1,263 @@
+/*
+ * fastcon.h — C ABI of libfastcon, the MI355X (gfx950) BFS and s–t
+ * connectivity engine.
+ *
+ * Plain C, plain pointers and sizes; no HIP or CUDA types cross this line.
+ * Every entry point returns an fc_status and never calls exit(); the text of
+ * the last error on the calling thread is available from fc_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   BFS-4K   class cudaGraph {cudaGraph(Graph&), Reset, cudaBFS4K, cudaBFS4K_N}
+ *            BFS-4K/include/device/cudaGraph.cuh:14-35
+ *   BFS-4K   class Graph (host CSR, host BFS oracle)
+ *            BFS-4K/include/host/graph.h:15-42, BFS-4K/src/host/graph.cpp:19-114
+ *   BFS-4K   readGraph::readGraphHeader / readSTD
+ *            BFS-4K/Util/readGraph.h:17-23
+ *   BFS_4K_DS double-source visit (found_common)
+ *            BFS_4K_DS/src/device/BFS_WorkEfficient.cu:8-98, BFS_4K_DS/src/device/cudaGraph.cu:5
+ *   ST-CON   extern "C" make_stcon(...)
+ *            bernaschi/ST_CONN_ONE_ATOMIC/current-st/st-conn/include/make_stcon.h:11-15
+ *   ST-CON   run_make_struct (symmetrise, drop self-loops and duplicates)
+ *            bernaschi/ST_CONN_ONE_ATOMIC/current-st/st-conn/make_struct.c:304-356,553-738
+ *   ST-CON   find_stcon_roots (seeded root sampling)
+ *            bernaschi/ST_CONN_ONE_ATOMIC/current-st/st-conn/reference_functions.c:483-527
+ *
+ * Encodings:
+ *   FC_INF (-1) marks an unreached vertex in level arrays. The reference
+ *   writes INF = (dist_t)-1 = 65535 (BFS-4K/config.h:14,50) into an int array;
+ *   a level of 65535 never occurs on the supported inputs, so the two
+ *   encodings map one to one (65535 <-> -1).
+ *   Parents: -1 for unreached vertices, the root points to itself (as the
+ *   ST-CON predecessor reconstruction does, st-conn/make_stcon.c:1446-1455).
+ *
+ * Threading: one fc_graph per host thread; each fc_graph owns one HIP stream
+ * (default priority) and its own device workspace. Calls are synchronous from
+ * the caller's point of view.
+ */
+#ifndef FASTCON_H
+#define FASTCON_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FC_INF (-1)
+#define FC_VERSION_MAJOR 0
+#define FC_VERSION_MINOR 1
+
+typedef enum fc_status {
+  FC_OK = 0,
+  FC_ERR_INVALID = 1,   /* bad argument (null pointer, vertex out of range, ...) */
+  FC_ERR_HIP = 2,       /* HIP runtime error */
+  FC_ERR_NOMEM = 3,     /* host or device allocation failed */
+  FC_ERR_IO = 4,        /* unreadable or non-regular file (BFS-4K/Util/readUtil.cpp:52-61) */
+  FC_ERR_FORMAT = 5,    /* graph format not recognised (BFS-4K/Util/readGraphHeader.cpp:37-38) */
+  FC_ERR_CAPACITY = 6,  /* a fixed-capacity buffer would overflow (BFS-4K/src/device/BFS_WorkEfficient.cu:111-112) */
+  FC_ERR_NOGPU = 7,     /* no HIP device, or the device code object is missing */
+  FC_ERR_TIMEOUT = 8,   /* a device-side bounded wait expired; results are invalid */
+  FC_ERR_UNSUPPORTED = 9
+} fc_status;
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* fc_last_error(void);
+/* "fastcon <major>.<minor> gfx950" */
+const char* fc_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host graphs (CSR in host memory). Built by the readers, the generators and */
+/* the canonicaliser below; released with fc_host_graph_free.                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fc_host_graph {
+  int64_t  V;          /* vertices */
+  int64_t  E;          /* directed edges = length of columns */
+  int64_t* offsets;    /* V+1 entries; adjacency of v is columns[offsets[v] .. offsets[v+1]) */
+  int32_t* columns;    /* E entries */
+  int32_t  directed;   /* 0: every edge is stored in both directions */
+  int32_t  reserved;
+} fc_host_graph;
+
+/* Direction selector for the file readers: as BFS-4K's -D / -U switches. */
+typedef enum fc_direction {
+  FC_DIR_FILE = 0,        /* take it from the file header, directed when undefined */
+  FC_DIR_DIRECTED = 1,    /* -D */
+  FC_DIR_UNDIRECTED = 2   /* -U */
+} fc_direction;
+
+void fc_host_graph_free(fc_host_graph* g);
+
+/* Read a graph file in any FileFormat.txt format (MatrixMarket, DIMACS9 /
+ * GTgraph, DIMACS10 / METIS, SNAP; PGSolver headers are recognised but have
+ * no body reader, as in the reference). Vertex numbering, direction rules and
+ * adjacency order follow readGraphHeader + readSTD + Graph::ToCSR
+ * (BFS-4K/Util/readGraphHeader.cpp:14-114, BFS-4K/Util/readGraphSTD.cpp:14-145,
+ * BFS-4K/src/host/graph.cpp:19-66): no deduplication, adjacency in file order. */
+fc_status fc_read_graph(const char* path, fc_direction dir, fc_host_graph* out);
+
+/* Read the ST-CON edge-list loader's format (st-conn/load_graph.c:20-104). */
+fc_status fc_read_edge_list(const char* path, int64_t** edges_out, int64_t* nedges_out,
+                            int64_t* nverts_out);
+void fc_free(void* p);
+
+/* Canonical undirected CSR from an edge list, with the semantics of ST-CON's
+ * make_struct (st-conn/make_struct.c:304-356,553-738): every (u,v) also adds
+ * (v,u); self-loops and edges with a negative endpoint are dropped; duplicate
+ * pairs are removed; each adjacency list is sorted ascending.
+ * edges: 2*m int64 values (u0,v0,u1,v1,...). V: vertex count (0 = max id + 1). */
+fc_status fc_csr_from_edges(const int64_t* edges, int64_t m, int64_t V, fc_host_graph* out);
+
+/* Seeded generators (this engine's own; deterministic for a seed at any
+ * thread count). All return canonical undirected CSR.
+ * R-MAT: 2^scale vertices, edgefactor*2^scale generated pairs, quadrant
+ * probabilities a, b, c and d = 1-a-b-c, vertex ids scrambled by a seeded
+ * bijection of [0, 2^scale). */
+fc_status fc_gen_rmat(int scale, double edgefactor, double a, double b, double c,
+                      uint64_t seed, fc_host_graph* out);
+/* rows x cols 4-neighbour grid, vertex r*cols+c; each undirected edge is
+ * deleted with probability p_delete. */
+fc_status fc_gen_grid(int32_t rows, int32_t cols, double p_delete, uint64_t seed,
+                      fc_host_graph* out);
+/* Erdős–Rényi G(n, M) with M = n*mean_degree/2 uniformly drawn pairs. */
+fc_status fc_gen_er(int64_t n, double mean_degree, uint64_t seed, fc_host_graph* out);
+
+/* n distinct seeded vertices of degree > 0 (the acceptance rule of
+ * find_stcon_roots, st-conn/reference_functions.c:483-527, drawn from this
+ * engine's own stream). Returns FC_ERR_INVALID if fewer exist. */
+fc_status fc_sample_roots(const fc_host_graph* g, int32_t n, uint64_t seed, int32_t* out);
+
+/* Connected-component labels (label = smallest vertex id of the component). */
+fc_status fc_components(const fc_host_graph* g, int32_t* label_out);
+
+/* ------------------------------------------------------------------------ */
+/* Device graphs and queries.                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fc_graph fc_graph;
+
+/* Upload a CSR graph; the library keeps device copies and allocates all
+ * per-query workspace once. Replaces cudaGraph::cudaGraph
+ * (BFS-4K/src/device/cudaGraph.cu:7-36). flags: reserved, pass 0. */
+fc_status fc_graph_create_csr32(const int32_t* off, const int32_t* col, int32_t V, int64_t E,
+                                uint32_t flags, fc_graph** out);
+fc_status fc_graph_create_csr64(const int64_t* off, const int32_t* col, int64_t V, int64_t E,
+                                uint32_t flags, fc_graph** out);
+fc_status fc_graph_destroy(fc_graph* g);
+
+/* Per-call statistics. */
+typedef struct fc_stats {
+  double  time_ms;            /* device time of the visit (HIP events on the graph's stream) */
+  int32_t levels;             /* BFS: number of non-empty levels; s–t: visit iterations */
+  int32_t depth;              /* s–t: iteration at which the frontiers met (ST-CON DEPTH) */
+  int64_t visited_vertices;   /* BFS: reached vertices V_r; s–t: labelled vertices */
+  int64_t visited_edges;      /* BFS: sum of degrees of reached vertices E_r */
+  int64_t expanded_vertices;  /* frontier vertices whose adjacency was scanned */
+  int64_t expanded_edges;     /* sum of degrees of the expanded vertices */
+  int32_t topdown_levels;
+  int32_t bottomup_levels;
+  int32_t max_frontier;
+  int32_t reserved;
+} fc_stats;
+
+/* Single-source BFS. Replaces Reset + cudaBFS4K + readback
+ * (BFS-4K/src/device/BFS_WorkEfficient.cu:81-133).
+ * levels_out / parents_out: host arrays of V entries, or NULL to leave the
+ * result in device memory (fetch later with fc_graph_levels / fc_graph_parents).
+ * Levels equal Graph::bfs Distance[] (BFS-4K/src/host/graph.cpp:83-97). */
+fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32_t* parents_out,
+                 fc_stats* stats);
+fc_status fc_graph_levels(fc_graph* g, int32_t* levels_out);
+fc_status fc_graph_parents(fc_graph* g, int32_t* parents_out);
+
+/* Plain s–t connectivity: one query per launch, two colours, one atomicCAS
+ * per vertex, both frontiers expanded each iteration, stop when they meet
+ * (ST-CON make_stcon, st-conn/make_stcon.c:730-1497, and the BFS_4K_DS
+ * visit, BFS_4K_DS/src/device/WorkEfficientKernel/BFS_WE_Kernels1.cu:25-89).
+ * connected: 1/0. dist: exact hop distance (Graph::bfs level of t from s),
+ * -1 if not connected. meet: a vertex on a shortest path where the two
+ * colours met, or -1 (nullable). stats->depth is ST-CON's DEPTH = ceil(dist/2). */
+fc_status fc_stcon(fc_graph* g, int32_t s, int32_t t, int32_t* connected, int32_t* dist,
+                   int32_t* meet, fc_stats* stats);
+
+/* Many plain queries, each still one launch (the reference's per-pair loop,
+ * st-conn/make_stcon.c:418-452). */
+fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
+                        int32_t* connected, int32_t* dist, fc_stats* stats);
+
+/* Plain, heuristic or learned scheduling of one-query-per-launch visits
+ * (policy: fc_policy). FC_POLICY_PLAIN is fc_stcon_many. The heuristic and
+ * learned policies expand one side per step (top-down or bottom-up) and give
+ * the same verdicts and distances. */
+fc_status fc_stcon_sched_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy,
+                              int32_t* connected, int32_t* dist, fc_stats* stats);
+
+/* Scheduling policy of a visit step. */
+typedef enum fc_policy {
+  FC_POLICY_PLAIN = 0,      /* both colours, top-down, every iteration (reference semantics) */
+  FC_POLICY_HEURISTIC = 1,  /* smaller side first; direction and width from frontier statistics */
+  FC_POLICY_LEARNED = 2     /* the trained MLP picks side, direction and width; heuristic fallback */
+} fc_policy;
+
+typedef struct fc_batch_opts {
+  int32_t policy;        /* fc_policy */
+  int32_t vw;            /* lanes per vertex for top-down expansion (4..64); 0 = autotuned */
+  int32_t tile;          /* tile schedule (work items per wave-iteration); 0 = autotuned */
+  int32_t use_graph;     /* 1: replay a captured hipGraph per wave; 0: plain launches */
+  int32_t words;         /* 64-bit query words per vertex (1 = 64 queries per wave) */
+  int32_t reserved[3];
+} fc_batch_opts;
+
+typedef struct fc_batch_stats {
+  double  time_ms;
+  int32_t waves;
+  int32_t levels;              /* sum over waves of visit steps */
+  int64_t active_vertices;     /* sum over steps of vertices with a non-zero frontier word */
+  int64_t active_edges;        /* sum over steps of their degrees */
+  int64_t touched_vertices;    /* sum over waves of vertices whose masks were written */
+  int32_t vw;                  /* width used */
+  int32_t tile;                /* tile schedule used */
+} fc_batch_stats;
+
+/* Batched s–t: queries are run as bit-parallel waves of 64 (one uint64 mask
+ * per vertex and side). Verdicts and distances equal fc_stcon's. */
+fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
+                         int32_t* connected, int32_t* dist, const fc_batch_opts* opts,
+                         fc_batch_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* Learned scheduler: 12 frontier features -> 32 -> 32 -> 6 choices, FP32.   */
+/* ------------------------------------------------------------------------ */
+
+#define FC_SCHED_FEATURES 12
+#define FC_SCHED_HIDDEN 32
+#define FC_SCHED_ACTIONS 6
+#define FC_SCHED_PARAMS (12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6)
+
+/* Per-step trace record logged by timing every alternative from one state. */
+typedef struct fc_trace_row {
+  float features[FC_SCHED_FEATURES];
+  float cost_us[FC_SCHED_ACTIONS];   /* measured time of each alternative, microseconds */
+} fc_trace_row;
+
+/* Run queries step by step, timing all FC_SCHED_ACTIONS alternatives at each
+ * step, and append up to cap rows. Returns the number written in *n_rows. */
+fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
+                         fc_trace_row* rows, int32_t cap, int32_t* n_rows);
+/* Train on the device (hand-written forward/backward/Adam kernels). Loss is
+ * the mean squared error of log-cost predictions. */
+fc_status fc_sched_train(fc_graph* g, const fc_trace_row* rows, int32_t n_rows, int32_t epochs,
+                         float lr, uint64_t seed, float* final_loss);
+fc_status fc_sched_set_params(fc_graph* g, const float* params, int32_t n);
+fc_status fc_sched_get_params(fc_graph* g, float* params, int32_t n);
+/* One forward pass (and optionally the gradient of 0.5*sum((y-target)^2)/n
+ * w.r.t. params) on the device; used to check the kernels against a CPU oracle. */
+fc_status fc_sched_forward(fc_graph* g, const float* x, int32_t n, float* y_out);
+fc_status fc_sched_gradient(fc_graph* g, const float* x, const float* target, int32_t n,
+                            float* loss_out, float* grad_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTCON_H */

