@@ -7,9 +7,16 @@
 // degree >= hub_threshold) instead of the reference's free/2 frontier buffers.
 // fc_bfs replaces Reset + cudaBFS4K (BFS-4K/src/device/BFS_WorkEfficient.cu:81-133);
 // fc_stcon replaces make_stcon's single-GPU path (st-conn/make_stcon.c:730-1497).
+//
+// Queries run through the device-side phase machine of visit.hip: the host
+// writes the query list and an initial control block, then replays a
+// captured hipGraph of identical phase launches (or launches them directly
+// when only a few are needed), reading back the 8-byte phase word between
+// batches to know whether every query has finished.
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -28,19 +35,35 @@ fc_status hip_fail(hipError_t e, const char* what) {
               "%s: %s", what, hipGetErrorString(e));
 }
 
-fc_status ensure_out(fc_graph* g, int32_t n) {
-  if (n <= g->out_cap) return FC_OK;
-  if (g->h_out) FC_HIP(hipHostFree(g->h_out));
+static void drop_graph(fc_graph* g) {
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+}
+
+fc_status ensure_queries(fc_graph* g, int32_t n) {
+  if (n <= g->q_cap) return FC_OK;
+  drop_graph(g);  // the captured launches hold the old pointers
+  (void)hipFree(g->d_qs);
+  (void)hipFree(g->d_qt);
+  (void)hipFree(g->d_out);
+  if (g->h_out) (void)hipHostFree(g->h_out);
+  g->d_qs = g->d_qt = nullptr;
+  g->d_out = nullptr;
   g->h_out = nullptr;
-  g->out_cap = 0;
-  int32_t cap = std::max<int32_t>(n, 64);
+  g->q_cap = 0;
+  const int32_t cap = std::max<int32_t>(n, 1024);
+  FC_HIP(hipMalloc((void**)&g->d_qs, sizeof(int32_t) * (size_t)cap));
+  FC_HIP(hipMalloc((void**)&g->d_qt, sizeof(int32_t) * (size_t)cap));
+  FC_HIP(hipMalloc((void**)&g->d_out, sizeof(VisitOut) * (size_t)cap));
   FC_HIP(hipHostMalloc((void**)&g->h_out, sizeof(VisitOut) * (size_t)cap, hipHostMallocDefault));
-  g->out_cap = cap;
+  g->q_cap = cap;
   return FC_OK;
 }
 
 template <typename OffT>
-static VisitParams<OffT> make_params(fc_graph* g) {
+static VisitParams<OffT> make_params(fc_graph* g, int force_action) {
   VisitParams<OffT> p;
   std::memset(&p, 0, sizeof(p));
   p.off = (const OffT*)g->d_off;
@@ -49,7 +72,7 @@ static VisitParams<OffT> make_params(fc_graph* g) {
   p.E = g->E;
   p.hub_threshold = g->hub_threshold;
   p.state = g->d_state;
-  p.parent = nullptr;
+  p.parent = g->d_parent;
   p.visited = g->d_visited;
   for (int c = 0; c < 2; ++c) {
     for (int k = 0; k < 3; ++k) p.fbm[c][k] = g->d_fbm[c][k];
@@ -62,59 +85,111 @@ static VisitParams<OffT> make_params(fc_graph* g) {
   p.qcap = g->qcap;
   p.hcap = g->hcap;
   p.ctl = g->d_ctl;
-  p.spin_limit = g->spin_limit;
-  p.force_side = -1;
-  p.force_strategy = -1;
-  p.trace_slot = -1;
+  p.qs = g->d_qs;
+  p.qt = g->d_qt;
+  p.out = g->d_out;
+  p.mlp = g->d_mlp;
+  p.force_action = force_action;
   return p;
 }
 
-fc_status launch_one(fc_graph* g, int mode, int policy, int32_t src, int32_t dst, bool parents, int slot,
-                     int force_side, int force_strategy, int max_steps, bool resume, int lvl0, int lvl1,
-                     float* trace_feat) {
-  hipError_t e;
-  if (g->off64) {
-    VisitParams<int64_t> p = make_params<int64_t>(g);
-    p.parent = parents ? g->d_parent : nullptr;
-    p.out = g->h_out + slot;
-    p.src = src;
-    p.dst = dst;
-    p.mode = mode;
-    p.policy = policy;
-    p.mlp = (policy == 2 && g->mlp_set) ? g->d_mlp : nullptr;
-    p.force_side = force_side;
-    p.force_strategy = force_strategy;
-    p.max_steps = max_steps;
-    p.resume = resume ? 1 : 0;
-    p.lvl0 = lvl0;
-    p.lvl1 = lvl1;
-    p.trace_feat = trace_feat;
-    e = launch_visit<int64_t>(p, g->blocks, g->stream);
-  } else {
-    VisitParams<int32_t> p = make_params<int32_t>(g);
-    p.parent = parents ? g->d_parent : nullptr;
-    p.out = g->h_out + slot;
-    p.src = src;
-    p.dst = dst;
-    p.mode = mode;
-    p.policy = policy;
-    p.mlp = (policy == 2 && g->mlp_set) ? g->d_mlp : nullptr;
-    p.force_side = force_side;
-    p.force_strategy = force_strategy;
-    p.max_steps = max_steps;
-    p.resume = resume ? 1 : 0;
-    p.lvl0 = lvl0;
-    p.lvl1 = lvl1;
-    p.trace_feat = trace_feat;
-    e = launch_visit<int32_t>(p, g->blocks, g->stream);
+static hipError_t launch_raw(fc_graph* g, int force_action) {
+  if (g->off64) return launch_phase<int64_t>(make_params<int64_t>(g, force_action), g->blocks, g->stream);
+  return launch_phase<int32_t>(make_params<int32_t>(g, force_action), g->blocks, g->stream);
+}
+
+fc_status launch_phase_once(fc_graph* g, int force_action) {
+  hipError_t e = launch_raw(g, force_action);
+  if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
+  return FC_OK;
+}
+
+static fc_status ensure_graph(fc_graph* g) {
+  if (g->gexec) return FC_OK;
+  FC_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = launch_raw(g, -1);
+  hipGraph_t gr = nullptr;
+  hipError_t ce = hipStreamEndCapture(g->stream, &gr);
+  if (le != hipSuccess) return hip_fail(le, "phase kernel capture");
+  if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+  g->graph = gr;
+  FC_HIP(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
+  return FC_OK;
+}
+
+fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const int32_t* t, int32_t n, bool parents,
+                     double* ms) {
+  *ms = 0.0;
+  if (n <= 0) return FC_OK;
+  fc_status st = ensure_queries(g, n);
+  if (st != FC_OK) return st;
+  st = ensure_graph(g);
+  if (st != FC_OK) return st;
+  FC_HIP(hipMemcpyAsync(g->d_qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+  FC_HIP(hipMemcpyAsync(g->d_qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+  VisitCtl init;
+  std::memset(&init, 0, sizeof(init));
+  init.phase = kPhaseReset;
+  init.qi = 0;
+  init.nq = n;
+  init.mode = mode;
+  init.policy = policy;
+  init.vw_fixed = g->vw_fixed;
+  init.record_parents = parents ? 1 : 0;
+  init.src = s[0];
+  init.dst = t[0];
+  init.ticket = 0;
+  init.best = ~0ull;
+  FC_HIP(hipMemcpyAsync(g->d_ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
+  // The async copies above read pageable host memory: finish them before the
+  // caller's arrays can go away, and before timing starts.
+  FC_HIP(hipStreamSynchronize(g->stream));
+
+  const double est = g->est_levels[mode];
+  int64_t expected = (int64_t)std::ceil((double)n * (est + 1.0)) + 1;
+  int64_t launched = 0;
+  FC_HIP(hipEventRecord(g->ev0, g->stream));
+  for (int round = 0;; ++round) {
+    int64_t want = expected - launched;
+    if (want < 1) want = 1;
+    if (want < kGraphLaunches) {
+      for (int64_t i = 0; i < want; ++i) {
+        hipError_t e = launch_raw(g, -1);
+        if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
+      }
+      launched += want;
+    } else {
+      const int64_t ng = (want + kGraphLaunches - 1) / kGraphLaunches;
+      for (int64_t i = 0; i < ng; ++i) FC_HIP(hipGraphLaunch(g->gexec, g->stream));
+      launched += ng * kGraphLaunches;
+    }
+    FC_HIP(hipEventRecord(g->ev1, g->stream));
+    FC_HIP(hipMemcpyAsync(g->h_phase, g->d_ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    FC_HIP(hipStreamSynchronize(g->stream));
+    const int32_t phase = g->h_phase[0], qi = g->h_phase[1];
+    if (phase == kPhaseIdle) break;
+    // Every query stops within V+2 levels (device guard), so this bound is
+    // never reached by a working visit.
+    if (launched > (int64_t)n * (g->V + 4) + 64)
+      return fail(FC_ERR_CAPACITY, "phase machine did not finish after %lld launches", (long long)launched);
+    // Not finished: budget the remaining queries with a wider margin.
+    const int64_t remaining = std::max<int64_t>(n - qi, 1);
+    expected = launched + (int64_t)std::ceil((double)remaining * (est + 2.0) * (1.0 + round)) + 1;
   }
-  if (e != hipSuccess) return hip_fail(e, "visit kernel launch");
+  float fms = 0.f;
+  FC_HIP(hipEventElapsedTime(&fms, g->ev0, g->ev1));
+  *ms = fms;
+  FC_HIP(hipMemcpyAsync(g->h_out, g->d_out, sizeof(VisitOut) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
+  FC_HIP(hipStreamSynchronize(g->stream));
+  double lv = 0;
+  for (int32_t i = 0; i < n; ++i) lv += g->h_out[i].levels;
+  g->est_levels[mode] = 0.5 * g->est_levels[mode] + 0.5 * (lv / n);
   return FC_OK;
 }
 
 }  // namespace fcd
 
-using fcd::ensure_out;
 using fcd::hip_fail;
 
 template <typename OffT>
@@ -131,7 +206,10 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   g->E = E;
   g->off64 = sizeof(OffT) == 8;
   const char* ht = std::getenv("FASTCON_HUB_THRESHOLD");
+  g->hub_threshold = 512;
   if (ht && std::atoi(ht) > 0) g->hub_threshold = std::atoi(ht);
+  const char* vwf = std::getenv("FASTCON_VW");
+  if (vwf && std::atoi(vwf) > 0) g->vw_fixed = std::atoi(vwf);
   int64_t hubs = 0;
   fc_status st = sizeof(OffT) == 8
                      ? fc::validate_csr64((const int64_t*)off, col, V, E, g->hub_threshold, &hubs)
@@ -144,9 +222,9 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
     fc_graph_destroy(g);
     return s;
   };
-#define CK(call)                                  \
-  do {                                            \
-    hipError_t _e = (call);                       \
+#define CK(call)                                            \
+  do {                                                      \
+    hipError_t _e = (call);                                 \
     if (_e != hipSuccess) return bail(hip_fail(_e, #call)); \
   } while (0)
   CK(hipGetDevice(&g->device));
@@ -191,18 +269,16 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   }
   g->d_ctl = (fcd::VisitCtl*)(g->d_ws + off_ctl);
   CK(hipMalloc((void**)&g->d_mlp, sizeof(float) * fcd::kMlpParams));
+  CK(hipMemset(g->d_mlp, 0, sizeof(float) * fcd::kMlpParams));
+  CK(hipHostMalloc((void**)&g->h_phase, 4 * sizeof(int32_t), hipHostMallocDefault));
 
-  int occ = sizeof(OffT) == 8 ? fcd::visit_occupancy<int64_t>() : fcd::visit_occupancy<int32_t>();
-  if (occ < 1) return bail(fail(FC_ERR_NOGPU, "visit kernel has no occupancy on this device (missing gfx950 code?)"));
-  int per_cu = std::min(occ, 4);
+  int occ = sizeof(OffT) == 8 ? fcd::phase_occupancy<int64_t>() : fcd::phase_occupancy<int32_t>();
+  if (occ < 1) return bail(fail(FC_ERR_NOGPU, "phase kernel has no occupancy on this device (missing gfx950 code?)"));
+  int per_cu = std::min(occ, 8);
   const char* bpc = std::getenv("FASTCON_BLOCKS_PER_CU");
   if (bpc && std::atoi(bpc) > 0) per_cu = std::min(occ, std::atoi(bpc));
   g->blocks = per_cu * g->cus;
-  // Barrier waits give up after ~10 s of real time (100 MHz counter).
-  g->spin_limit = 1000000000ull;
-  const char* sl = std::getenv("FASTCON_SPIN_LIMIT");
-  if (sl && std::atoll(sl) > 0) g->spin_limit = (uint64_t)std::atoll(sl);
-  fc_status s2 = ensure_out(g, 64);
+  fc_status s2 = fcd::ensure_queries(g, 1024);
   if (s2 != FC_OK) return bail(s2);
 #undef CK
   *out = g;
@@ -223,7 +299,13 @@ extern "C" fc_status fc_graph_destroy(fc_graph* g) {
   if (!g) return FC_OK;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   fcd::batch_free(g->batch);
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->h_out) (void)hipHostFree(g->h_out);
+  if (g->h_phase) (void)hipHostFree(g->h_phase);
+  (void)hipFree(g->d_qs);
+  (void)hipFree(g->d_qt);
+  (void)hipFree(g->d_out);
   (void)hipFree(g->d_mlp);
   (void)hipFree(g->d_snap);
   (void)hipFree(g->d_ws);
@@ -236,28 +318,19 @@ extern "C" fc_status fc_graph_destroy(fc_graph* g) {
   return FC_OK;
 }
 
-static void fill_stats(const fcd::VisitOut& o, double ms, fc_stats* s) {
-  if (!s) return;
-  std::memset(s, 0, sizeof(*s));
-  s->time_ms = ms;
-  s->levels = o.levels;
-  s->depth = o.depth;
-  s->visited_vertices = o.visited_vertices;
-  s->visited_edges = o.visited_edges;
-  s->expanded_vertices = o.expanded_vertices;
-  s->expanded_edges = o.expanded_edges;
-  s->topdown_levels = o.topdown;
-  s->bottomup_levels = o.bottomup;
-  s->max_frontier = o.max_frontier;
+static void add_stats(const fcd::VisitOut& o, fc_stats* s) {
+  s->levels += o.levels;
+  s->depth += o.depth;
+  s->visited_vertices += o.visited_vertices;
+  s->visited_edges += o.visited_edges;
+  s->expanded_vertices += o.expanded_vertices;
+  s->expanded_edges += o.expanded_edges;
+  s->topdown_levels += o.topdown;
+  s->bottomup_levels += o.bottomup;
+  s->max_frontier = std::max(s->max_frontier, o.max_frontier);
 }
 
-static fc_status check_out(fc_graph* g, const fcd::VisitOut& o) {
-  if (o.status == fcd::FC_TIMEOUT_STATUS) {
-    // Leave the barrier usable for the next launch.
-    (void)hipMemsetAsync(g->d_ctl, 0, sizeof(fcd::GridBar), g->stream);
-    (void)hipStreamSynchronize(g->stream);
-    return fail(FC_ERR_TIMEOUT, "visit kernel: grid barrier wait exceeded its bound; results are invalid");
-  }
+static fc_status check_out(const fcd::VisitOut& o) {
   if (o.status != 0) return fail(FC_ERR_CAPACITY, "visit kernel: frontier guard tripped (err=%d)", o.err);
   return FC_OK;
 }
@@ -267,19 +340,21 @@ extern "C" fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32
   if (!g) return fail(FC_ERR_INVALID, "fc_bfs: null graph");
   if (src < 0 || src >= g->V) return fail(FC_ERR_INVALID, "fc_bfs: source %d out of range [0,%lld)", src, (long long)g->V);
   const bool par = parents_out != nullptr;
-  FC_HIP(hipEventRecord(g->ev0, g->stream));
-  fc_status st = fcd::launch_one(g, fcd::kModeBfs, 0, src, -1, par, 0);
+  const int32_t none = -1;
+  double ms = 0;
+  g->levels_valid = g->parents_valid = false;
+  fc_status st = fcd::run_visits(g, fcd::kModeBfs, 0, &src, &none, 1, par, &ms);
   if (st != FC_OK) return st;
-  FC_HIP(hipEventRecord(g->ev1, g->stream));
-  FC_HIP(hipEventSynchronize(g->ev1));
-  float ms = 0.f;
-  FC_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
   const fcd::VisitOut o = g->h_out[0];
-  st = check_out(g, o);
-  g->levels_valid = st == FC_OK;
-  g->parents_valid = st == FC_OK && par;
+  st = check_out(o);
   if (st != FC_OK) return st;
-  fill_stats(o, ms, stats);
+  g->levels_valid = true;
+  g->parents_valid = par;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->time_ms = ms;
+    add_stats(o, stats);
+  }
   if (levels_out) FC_HIP(hipMemcpy(levels_out, g->d_state, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
   if (parents_out) FC_HIP(hipMemcpy(parents_out, g->d_parent, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
   return FC_OK;
@@ -309,47 +384,40 @@ static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
   for (int32_t i = 0; i < n; ++i)
     if (s[i] < 0 || s[i] >= g->V || t[i] < 0 || t[i] >= g->V)
       return fail(FC_ERR_INVALID, "s–t query %d: vertex out of range [0,%lld)", i, (long long)g->V);
-  fc_status st = ensure_out(g, n);
-  if (st != FC_OK) return st;
   g->levels_valid = g->parents_valid = false;
-  FC_HIP(hipEventRecord(g->ev0, g->stream));
-  for (int32_t i = 0; i < n; ++i) {
-    if (s[i] == t[i]) continue;
-    st = fcd::launch_one(g, mode, policy, s[i], t[i], false, i);
-    if (st != FC_OK) return st;
-  }
-  FC_HIP(hipEventRecord(g->ev1, g->stream));
-  FC_HIP(hipEventSynchronize(g->ev1));
-  float ms = 0.f;
-  FC_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-  fcd::VisitOut agg;
-  std::memset(&agg, 0, sizeof(agg));
-  for (int32_t i = 0; i < n; ++i) {
-    fcd::VisitOut o;
-    if (s[i] == t[i]) {
-      std::memset(&o, 0, sizeof(o));
-      o.dist = 0;
-      o.meet = s[i];
-      o.visited_vertices = 1;
-    } else {
-      o = g->h_out[i];
-      st = check_out(g, o);
-      if (st != FC_OK) return st;
+  // s == t needs no visit: distance 0.
+  std::vector<int32_t> vs, vt, idx;
+  vs.reserve(n);
+  vt.reserve(n);
+  idx.reserve(n);
+  for (int32_t i = 0; i < n; ++i)
+    if (s[i] != t[i]) {
+      vs.push_back(s[i]);
+      vt.push_back(t[i]);
+      idx.push_back(i);
     }
+  double ms = 0;
+  fc_status st = fcd::run_visits(g, mode, policy, vs.data(), vt.data(), (int32_t)vs.size(), false, &ms);
+  if (st != FC_OK) return st;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->time_ms = ms;
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    if (connected) connected[i] = 1;
+    if (dist) dist[i] = 0;
+    if (meet) meet[i] = s[i];
+  }
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const fcd::VisitOut o = g->h_out[k];
+    st = check_out(o);
+    if (st != FC_OK) return st;
+    const int32_t i = idx[k];
     if (connected) connected[i] = o.dist >= 0 ? 1 : 0;
     if (dist) dist[i] = o.dist;
     if (meet) meet[i] = o.meet;
-    agg.levels += o.levels;
-    agg.depth += o.depth;
-    agg.visited_vertices += o.visited_vertices;
-    agg.visited_edges += o.visited_edges;
-    agg.expanded_vertices += o.expanded_vertices;
-    agg.expanded_edges += o.expanded_edges;
-    agg.topdown += o.topdown;
-    agg.bottomup += o.bottomup;
-    agg.max_frontier = std::max(agg.max_frontier, o.max_frontier);
+    if (stats) add_stats(o, stats);
   }
-  fill_stats(agg, ms, stats);
   return FC_OK;
 }
 
@@ -368,10 +436,12 @@ extern "C" fc_status fc_stcon_sched_many(fc_graph* g, const int32_t* s, const in
   if (policy == FC_POLICY_PLAIN) return fc_stcon_many(g, s, t, n, connected, dist, stats);
   if (policy != FC_POLICY_HEURISTIC && policy != FC_POLICY_LEARNED)
     return fail(FC_ERR_INVALID, "unknown policy %d", policy);
-  // One-sided scheduled steps inside the persistent kernel are switched off:
-  // on the GPU such a launch waited out its grid-barrier bound and the cause
-  // has not been identified. Scheduled visits run in the batched engine.
+  // One-sided scheduled steps for single queries stay switched off: an
+  // earlier persistent-kernel version of this path waited out its
+  // grid-barrier bound on the GPU and that cause was not identified.
+  // Heuristic and learned scheduling run in the batched engine.
+  (void)stcon_run;
   return fail(FC_ERR_UNSUPPORTED,
-              "scheduled single-query visits are disabled (grid-barrier timeout under investigation); "
-              "use fc_stcon_batch for heuristic or learned scheduling");
+              "scheduled single-query visits are switched off; use fc_stcon_batch for heuristic or "
+              "learned scheduling");
 }

@@ -42,8 +42,19 @@ struct fc_graph {
   int64_t* d_hb[2][2] = {};
   fcd::VisitCtl* d_ctl = nullptr;
 
-  fcd::VisitOut* h_out = nullptr;   // pinned, out_cap entries
-  int32_t out_cap = 0;
+  // Query lists and results of the phase machine (device), with a pinned
+  // host copy of the results; capacity q_cap.
+  int32_t* d_qs = nullptr;
+  int32_t* d_qt = nullptr;
+  fcd::VisitOut* d_out = nullptr;
+  fcd::VisitOut* h_out = nullptr;
+  int32_t q_cap = 0;
+  int32_t* h_phase = nullptr;       // pinned: phase + qi read back between batches
+  // Captured hipGraph of kGraphLaunches identical phase launches.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double est_levels[3] = {12.0, 4.0, 4.0};  // running estimate per mode
+  int32_t vw_fixed = 0;
 
   bool parents_valid = false;
   bool levels_valid = false;
@@ -52,23 +63,26 @@ struct fc_graph {
   std::vector<float> h_mlp;
 
   fcd::BatchWorkspace* batch = nullptr;
-  uint64_t spin_limit = 0;          // 100 MHz ticks
 };
 
 namespace fcd {
 
+constexpr int kGraphLaunches = 16;
+
 fc_status hip_fail(hipError_t e, const char* what);
-fc_status ensure_out(fc_graph* g, int32_t n);
+fc_status ensure_queries(fc_graph* g, int32_t n);
 
-// Launch one visit (cooperative) writing into g->h_out[slot]; no sync.
-fc_status launch_one(fc_graph* g, int mode, int policy, int32_t src, int32_t dst, bool parents, int slot,
-                     int force_side = -1, int force_strategy = -1, int max_steps = 0, bool resume = false,
-                     int lvl0 = 0, int lvl1 = 0, float* trace_feat = nullptr);
+// Run n visits through the phase machine (s[i], t[i]; t = -1 for BFS) and
+// leave their results in g->h_out[0..n). *ms receives the device time.
+fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const int32_t* t, int32_t n,
+                     bool parents, double* ms);
+// One phase launch with an optional forced scheduler action (tracing).
+fc_status launch_phase_once(fc_graph* g, int force_action);
 
 template <typename OffT>
-hipError_t launch_visit(const VisitParams<OffT>& p, int blocks, hipStream_t s);
+hipError_t launch_phase(const VisitParams<OffT>& p, int blocks, hipStream_t s);
 template <typename OffT>
-int visit_occupancy();
+int phase_occupancy();
 
 void batch_free(BatchWorkspace* b);
 

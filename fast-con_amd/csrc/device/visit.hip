@@ -1,31 +1,42 @@
-// Persistent level-loop visit kernel for BFS and s–t connectivity on gfx950.
+// Visit engine for BFS and s–t connectivity on gfx950: one kernel launch per
+// phase (the reset of a query, or one level of it), driven by a device-side
+// phase machine in VisitCtl.
 //
-// One cooperative launch runs a whole query: reset, every level, and the
-// result write-back. Between levels all blocks meet at a bounded grid barrier
-// (wave.hpp) instead of returning to the host, which replaces the reference's
-// per-level launch + 16-byte readback loop (BFS-4K/src/device/BFS_WorkEfficient.cu:81-118)
+// Why launches and not a persistent kernel: on MI355X a grid-wide software
+// barrier with the agent-scope release/acquire that cross-XCD visibility
+// needs costs 16-35 us, a cooperative launch ~20 us, while a back-to-back
+// ordinary launch costs ~2.4 us (1.6 us inside a hipGraph) and its boundary
+// already orders every block's writes. So each launch does one phase with all
+// blocks, and the last block to finish (an atomic ticket, no waiting) takes
+// the decisions for the next launch: direction, virtual-warp width and side,
+// termination, result write-back, and the move to the next query. The host
+// replays a captured hipGraph of identical launches; launches after the last
+// query return at once. This replaces the reference's host-driven level loop
+// with a 16-byte readback per level (BFS-4K/src/device/BFS_WorkEfficient.cu:81-118)
 // and ST-CON's ~12 host round trips per level (st-conn/make_stcon.c:968-1187).
 //
-// Frontier expansion (per colour, per step):
+// Frontier expansion (per colour, per level):
 //  * top-down: normal-queue vertices are expanded by virtual warps of VW lanes
-//    (4..64, chosen from the frontier's average degree), the reference's
+//    (4..64, from the frontier's average degree): the reference's
 //    BFS_KernelMainGLOB + EdgeVisit (BFS-4K/src/device/WorkEfficientKernel/BFS_WE_KernelMain.cu:2-53,
-//    BFS_WE_Kernels1.cu:44-74) rebuilt for 64-lane waves; vertices of degree >=
-//    hub_threshold are routed at claim time to a hub queue whose edges are
-//    spread over the whole grid, the device-side work queue that replaces the
-//    dynamic-parallelism child grids (BFS-4K/src/device/WorkEfficientKernel/BFS_WE_Dynamic.cu:4-50).
-//  * bottom-up: each wave owns 64 vertices, unvisited ones look for a parent in
-//    the current frontier bitmap (direction-optimising BFS, Beamer et al. SC'12).
+//    BFS_WE_Kernels1.cu:44-74) rebuilt for 64-lane waves. Vertices of degree >=
+//    hub_threshold are routed at claim time to a hub queue whose concatenated
+//    adjacency is cut into chunks spread over the whole grid: the device-side
+//    work queue that replaces the dynamic-parallelism child grids
+//    (BFS-4K/src/device/WorkEfficientKernel/BFS_WE_Dynamic.cu:4-50).
+//  * bottom-up: each wave owns 64 vertices; unvisited ones look for a parent
+//    in the current frontier bitmap (direction-optimising BFS, Beamer et al. SC'12).
 // Frontier slots are reserved with one atomic per wave (64-bit ballot + mbcnt),
 // the in-house replacement of FrontierWrite::Write (BFS-4K/src/device/Util/GlobalWrite.cu:49-105)
-// and of ST-CON's Thrust exclusive_scan/remove_copy compaction
+// and of ST-CON's Thrust exclusive_scan / remove_copy compaction
 // (st-conn/mythrustlib.cu:25-34,175-185), which cost O(V) per level.
 //
 // s–t: the state word is colour | level; a vertex is claimed with ONE atomicCAS
 // (ST-CON k_stcon_binary_mask_unique_large, st-conn/cudakernel/stcon_kernel.cu:128-196).
-// A CAS that returns the other colour is a meeting edge (v,u) and proposes the
-// path length level(v)+1+level(u); the visit ends after the step in which the
-// first proposal appears and the minimum proposal is the exact distance.
+// A CAS (or read) that shows the other colour is a meeting edge (v,u) and
+// proposes the path length level(v)+1+level(u); the visit ends after the level
+// in which the first proposal appears, and the minimum proposal is the exact
+// distance.
 #include "visit.hpp"
 
 #include <hip/hip_runtime.h>
@@ -50,10 +61,10 @@ __device__ __forceinline__ bool test_bit(const uint32_t* bm, int32_t v) {
 // Probe edge (v -> u) for colour c whose frontier is at level lvl. Returns
 // true when this lane claimed u for the next frontier.
 template <typename OffT>
-__device__ __forceinline__ bool probe_td(const VisitParams<OffT>& p, int c, int lvl, int32_t v,
-                                         int32_t u, bool act) {
+__device__ __forceinline__ bool probe_td(const VisitParams<OffT>& p, bool st, int c, int lvl, int32_t v, int32_t u,
+                                         bool act) {
   if (!act) return false;
-  if (p.mode == kModeBfs) {
+  if (!st) {
     // BFS claims through the visited bitmap: one atomicOr per newly reached
     // vertex; the plain pre-load filters most already-visited probes.
     const uint32_t bit = 1u << (u & 31);
@@ -118,7 +129,7 @@ __device__ __forceinline__ void append_next(const VisitParams<OffT>& p, int c, i
 }
 
 template <typename OffT, int VW>
-__device__ void td_normal(const VisitParams<OffT>& p, int c, int lvl, int cur_par, int nxt_par, uint32_t n,
+__device__ void td_normal(const VisitParams<OffT>& p, bool st, int c, int lvl, int cur_par, int nxt_par, uint32_t n,
                           Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
   constexpr int G = kWave / VW;
   const int lane = lane_id();
@@ -141,7 +152,7 @@ __device__ void td_normal(const VisitParams<OffT>& p, int c, int lvl, int cur_pa
     while (ballot(b < e)) {
       const bool act = b < e;
       const int32_t u = act ? p.col[b] : 0;
-      const bool cl = probe_td(p, c, lvl, v, u, act);
+      const bool cl = probe_td(p, st, c, lvl, v, u, act);
       append_next(p, c, nxt_par, nxt, nfbm, cl, u, mf_acc, true);
       b += VW;
     }
@@ -153,8 +164,8 @@ __device__ void td_normal(const VisitParams<OffT>& p, int c, int lvl, int cur_pa
 constexpr int kHubChunk = 256;
 
 template <typename OffT>
-__device__ void td_hubs(const VisitParams<OffT>& p, int c, int lvl, int cur_par, int nxt_par, uint64_t hubpk,
-                        Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
+__device__ void td_hubs(const VisitParams<OffT>& p, bool st, int c, int lvl, int cur_par, int nxt_par,
+                        uint64_t hubpk, Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
   uint32_t nh = (uint32_t)(hubpk >> kHubEdgeBits);
   if (nh > p.hcap) nh = p.hcap;
   const uint64_t H = hubpk & kHubEdgeMask;
@@ -174,26 +185,26 @@ __device__ void td_hubs(const VisitParams<OffT>& p, int c, int lvl, int cur_par,
     for (int k = 0; k < kHubChunk; k += kWave) {
       const uint64_t g = cb + (uint64_t)k + (uint64_t)lane;
       const bool act = g < H;
-      int32_t v = 0, u = 0;
+      int32_t v = -1, u = 0;
       if (act) {
         uint32_t s = lo;
         while (s + 1 < nh && (uint64_t)hb[s + 1] <= g) ++s;
         v = hq[s];
         bool good = (uint32_t)v < (uint32_t)p.V && (uint64_t)hb[s] <= g;
-        OffT k = 0;
+        OffT k2 = 0;
         if (good) {
-          k = p.off[v] + (OffT)(g - (uint64_t)hb[s]);
-          good = k < p.off[v + 1];
+          k2 = p.off[v] + (OffT)(g - (uint64_t)hb[s]);
+          good = k2 < p.off[v + 1];
         }
-        if (good)
-          u = p.col[k];
-        else {
+        if (good) {
+          u = p.col[k2];
+        } else {
           atomicOr(&p.ctl->err, 2u);
           v = -1;
         }
       }
-      const bool ok = act && v >= 0 && (uint32_t)v < (uint32_t)p.V;
-      const bool cl = probe_td(p, c, lvl, v, u, ok);
+      const bool ok = act && v >= 0;
+      const bool cl = probe_td(p, st, c, lvl, v, u, ok);
       append_next(p, c, nxt_par, nxt, nfbm, cl, u, mf_acc, true);
     }
   }
@@ -204,11 +215,12 @@ __device__ void td_hubs(const VisitParams<OffT>& p, int c, int lvl, int cur_par,
 // current frontier are scanned too, so that a frontier-to-frontier edge is
 // still seen as a meeting edge.
 template <typename OffT>
-__device__ void bu_step(const VisitParams<OffT>& p, int c, int lvl, const uint32_t* cfbm, const uint32_t* ofbm,
-                        uint32_t* nfbm, int nxt_par, Ring* nxt, uint64_t& mf_acc, int gwave, int nwaves) {
+__device__ void bu_step(const VisitParams<OffT>& p, bool st, int c, int lvl, const uint32_t* cfbm,
+                        const uint32_t* ofbm, uint32_t* nfbm, int nxt_par, Ring* nxt, uint64_t& mf_acc, int gwave,
+                        int nwaves) {
   const int lane = lane_id();
   const uint32_t nw64 = ((uint32_t)p.V + 63u) >> 6;
-  const int32_t mine = p.mode == kModeBfs ? lvl + 1 : ((c ? kColorBit : 0) | (lvl + 1));
+  const int32_t mine = st ? ((c ? kColorBit : 0) | (lvl + 1)) : lvl + 1;
   for (uint32_t w = (uint32_t)gwave; w < nw64; w += (uint32_t)nwaves) {
     const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
     const bool inr = v < p.V;
@@ -224,12 +236,20 @@ __device__ void bu_step(const VisitParams<OffT>& p, int c, int lvl, const uint32
     const bool small = cand && (e - b) <= (OffT)32;
     int32_t par = -1;
     if (small) {
-      for (OffT j = b; j < e; ++j) {
-        const int32_t u = p.col[j];
-        if (test_bit(cfbm, u)) {
-          par = u;
-          break;
-        }
+      // four independent column loads per round keep more requests in flight
+      for (OffT j = b; j < e && par < 0; j += 4) {
+        const int32_t u0 = p.col[j];
+        const int32_t u1 = j + 1 < e ? p.col[j + 1] : -1;
+        const int32_t u2 = j + 2 < e ? p.col[j + 2] : -1;
+        const int32_t u3 = j + 3 < e ? p.col[j + 3] : -1;
+        if (test_bit(cfbm, u0))
+          par = u0;
+        else if (u1 >= 0 && test_bit(cfbm, u1))
+          par = u1;
+        else if (u2 >= 0 && test_bit(cfbm, u2))
+          par = u2;
+        else if (u3 >= 0 && test_bit(cfbm, u3))
+          par = u3;
       }
     }
     uint64_t big = ballot(cand && !small);
@@ -266,7 +286,7 @@ __device__ void bu_step(const VisitParams<OffT>& p, int c, int lvl, const uint32
       atomicMin(&p.ctl->best, (cnd << 32) | (uint32_t)v);
     }
     // Bitmap words of these 64 vertices belong to this wave during a
-    // bottom-up step, so they are updated without atomics.
+    // bottom-up level, so they are updated without atomics.
     const uint64_t cm = ballot(claimed);
     if (lane == 0 && (uint32_t)cm) {
       p.visited[2u * w] |= (uint32_t)cm;
@@ -281,18 +301,18 @@ __device__ void bu_step(const VisitParams<OffT>& p, int c, int lvl, const uint32
 }
 
 template <typename OffT>
-__device__ void td_dispatch(const VisitParams<OffT>& p, int vw, int c, int lvl, int cur_par, int nxt_par, uint32_t n,
-                            Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
+__device__ void td_dispatch(const VisitParams<OffT>& p, bool st, int vw, int c, int lvl, int cur_par, int nxt_par,
+                            uint32_t n, Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
   switch (vw) {
-    case 4: td_normal<OffT, 4>(p, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    case 8: td_normal<OffT, 8>(p, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    case 16: td_normal<OffT, 16>(p, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    case 32: td_normal<OffT, 32>(p, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    default: td_normal<OffT, 64>(p, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
+    case 4: td_normal<OffT, 4>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
+    case 8: td_normal<OffT, 8>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
+    case 16: td_normal<OffT, 16>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
+    case 32: td_normal<OffT, 32>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
+    default: td_normal<OffT, 64>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
   }
 }
 
-// Width for a top-down step: the smallest power of two >= the average degree
+// Width for a top-down level: the smallest power of two >= the average degree
 // of the normal-queue entries, clamped to [4, 64].
 __device__ __forceinline__ int degree_width(uint32_t n, uint64_t mf_normal) {
   if (n == 0) return 4;
@@ -336,293 +356,269 @@ __device__ int mlp_argmin(const float* w, const float* x) {
   return best;
 }
 
-__device__ __forceinline__ float flog2(double x) { return (float)__log2f((float)(x + 1.0)); }
-
-// Frontier features of a scheduled s–t step (shared with the host tracer).
-__device__ void step_features(float* f, const uint32_t n[2], const uint64_t mf[2], const uint64_t hubn[2], int lvl0,
-                              int lvl1, int64_t V, int64_t E, uint64_t seen_edges) {
-  const double mu = (double)E - (double)seen_edges;
-  f[0] = flog2((double)n[0]) / 32.f;
-  f[1] = flog2((double)mf[0]) / 40.f;
-  f[2] = flog2((double)n[1]) / 32.f;
-  f[3] = flog2((double)mf[1]) / 40.f;
-  f[4] = flog2(mu > 0 ? mu : 0) / 40.f;
-  f[5] = flog2((double)V) / 32.f;
-  f[6] = flog2((double)E) / 40.f;
-  f[7] = (float)(lvl0 < 64 ? lvl0 : 64) / 64.f;
-  f[8] = (float)(lvl1 < 64 ? lvl1 : 64) / 64.f;
-  f[9] = flog2(n[0] ? (double)mf[0] / n[0] : 0.0) / 20.f;
-  f[10] = flog2(n[1] ? (double)mf[1] / n[1] : 0.0) / 20.f;
-  f[11] = (float)(hubn[0] + hubn[1]) / (float)(n[0] + n[1] + 1);
-}
-
 // Beamer's switch thresholds for direction-optimising BFS.
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
 
+// Scheduler action a = side * 3 + k: k=0 top-down width 8, k=1 top-down
+// width 64, k=2 bottom-up.
+__device__ __forceinline__ void apply_action(Decision& d, int a) {
+  const int side = a / 3, k = a % 3;
+  d.ex[0] = side == 0;
+  d.ex[1] = side == 1;
+  d.strat[side] = k == 2 ? kBottomUp : kTopDown;
+  d.strat[1 - side] = kTopDown;
+  d.vw[side] = k == 1 ? 64 : 8;
+  d.vw[1 - side] = 8;
+}
+
+// Decision for the next level launch (one thread of the finalising block).
+__device__ void decide(VisitCtl* ctl, const float* mlp, int64_t V, int64_t E) {
+  Decision d;
+  d.ex[0] = d.ex[1] = 0;
+  d.strat[0] = d.strat[1] = kTopDown;
+  d.vw[0] = d.vw[1] = 8;
+  uint32_t cn[2];
+  uint64_t cmf[2], chub[2];
+  for (int c = 0; c < 2; ++c) {
+    const Ring* R = &ctl->ring[c][ctl->lvl[c] % 3];
+    cn[c] = ld_agent(&R->n);
+    cmf[c] = ld_agent(&R->mf);
+    chub[c] = ld_agent(&R->hub);
+  }
+  const double mu = (double)E - (double)ctl->seen_edges;
+  const int mode = ctl->mode;
+  auto width = [&](int c) {
+    return ctl->vw_fixed > 0 ? ctl->vw_fixed : degree_width(cn[c], cmf[c] - (chub[c] & kHubEdgeMask));
+  };
+  if (mode == kModeBfs) {
+    d.ex[0] = 1;
+    const uint64_t nf = (uint64_t)cn[0] + (chub[0] >> kHubEdgeBits);
+    if (!ctl->bu_mode) {
+      if ((double)cmf[0] > mu / kAlpha) ctl->bu_mode = 1;
+    } else if ((double)nf * kBeta < (double)V) {
+      ctl->bu_mode = 0;
+    }
+    d.strat[0] = ctl->bu_mode ? kBottomUp : kTopDown;
+    d.vw[0] = width(0);
+  } else if (mode == kModePlainSt) {
+    d.ex[0] = d.ex[1] = 1;
+    d.vw[0] = width(0);
+    d.vw[1] = width(1);
+  } else {
+    const uint32_t nf[2] = {cn[0] + (uint32_t)(chub[0] >> kHubEdgeBits), cn[1] + (uint32_t)(chub[1] >> kHubEdgeBits)};
+    const uint64_t hn[2] = {chub[0] >> kHubEdgeBits, chub[1] >> kHubEdgeBits};
+    step_features(ctl->feat, nf, cmf, hn, ctl->lvl[0], ctl->lvl[1], V, E, ctl->seen_edges);
+    const int side = cmf[0] <= cmf[1] ? 0 : 1;
+    const bool bu = (double)cmf[side] > mu / kAlpha;
+    int a = side * 3 + (bu ? 2 : (width(side) >= 16 ? 1 : 0));
+    if (ctl->policy == 2 && mlp) a = mlp_argmin(mlp, ctl->feat);
+    apply_action(d, a);
+  }
+  ctl->dec = d;
+}
+
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void visit_kernel(VisitParams<OffT> p) {
+__device__ void finalize_reset(const VisitParams<OffT>& p) {
+  VisitCtl* ctl = p.ctl;
+  const bool st = ctl->mode != kModeBfs;
+  uint64_t seen = 0;
+  for (int c = 0; c < 2; ++c)
+    for (int k = 0; k < 3; ++k) ctl->ring[c][k] = Ring{0u, 0u, 0ull, 0ull};
+  for (int c = 0; c < (st ? 2 : 1); ++c) {
+    const int32_t r = c ? ctl->dst : ctl->src;
+    const int64_t d = (int64_t)(p.off[r + 1] - p.off[r]);
+    Ring& R = ctl->ring[c][0];
+    if (d >= p.hub_threshold) {
+      p.hq[c][0][0] = r;
+      p.hb[c][0][0] = 0;
+      R.hub = (1ull << kHubEdgeBits) | (uint64_t)d;
+    } else {
+      p.q[c][0][0] = r;
+      R.n = 1u;
+    }
+    R.mf = (uint64_t)d;
+    seen += (uint64_t)d;
+  }
+  ctl->lvl[0] = ctl->lvl[1] = 0;
+  ctl->steps = ctl->td = ctl->bu = ctl->maxf = 0;
+  ctl->bu_mode = 0;
+  ctl->best = ~0ull;
+  ctl->err = 0u;
+  ctl->seen_edges = seen;
+  ctl->visited_v = st ? 2 : 1;
+  ctl->visited_e = (int64_t)seen;
+  ctl->expv = ctl->expe = 0;
+  decide(ctl, p.mlp, p.V, p.E);
+  ctl->phase = kPhaseLevel;
+}
+
+template <typename OffT>
+__device__ void finalize_level(const VisitParams<OffT>& p, const Decision& d) {
+  VisitCtl* ctl = p.ctl;
+  const bool st = ctl->mode != kModeBfs;
+  ctl->steps++;
+  bool empty = false;
+  for (int c = 0; c < 2; ++c) {
+    if (!d.ex[c]) continue;
+    const Ring* cur = &ctl->ring[c][ctl->lvl[c] % 3];
+    const uint32_t nf = cur->n + (uint32_t)(cur->hub >> kHubEdgeBits);
+    ctl->expv += nf;
+    ctl->expe += (int64_t)cur->mf;
+    if ((int32_t)nf > ctl->maxf) ctl->maxf = (int32_t)nf;
+    if (d.strat[c] == kTopDown)
+      ctl->td++;
+    else
+      ctl->bu++;
+    ctl->lvl[c]++;
+    const Ring* R = &ctl->ring[c][ctl->lvl[c] % 3];
+    const uint64_t nmf = ld_agent(&R->mf);
+    const uint32_t nn = ld_agent(&R->n) + (uint32_t)(ld_agent(&R->hub) >> kHubEdgeBits);
+    ctl->seen_edges += nmf;
+    ctl->visited_e += (int64_t)nmf;
+    ctl->visited_v += nn;
+    if (nn == 0) empty = true;
+  }
+  const unsigned long long best = ld_agent(&ctl->best);
+  const uint32_t err = ld_agent(&ctl->err);
+  const bool met = st && best != ~0ull;
+  const bool runaway = ctl->steps > p.V + 2;
+  if (met || empty || runaway || err) {
+    VisitOut o;
+    o.status = (err || runaway) ? FC_CAPACITY_STATUS : 0;
+    o.levels = ctl->steps;
+    o.depth = met ? ctl->steps : 0;
+    o.dist = met ? (int32_t)(best >> 32) : -1;
+    o.meet = met ? (int32_t)(uint32_t)best : -1;
+    o.topdown = ctl->td;
+    o.bottomup = ctl->bu;
+    o.max_frontier = ctl->maxf;
+    o.visited_vertices = ctl->visited_v;
+    o.visited_edges = ctl->visited_e;
+    o.expanded_vertices = ctl->expv;
+    o.expanded_edges = ctl->expe;
+    o.lvl_end[0] = ctl->lvl[0];
+    o.lvl_end[1] = ctl->lvl[1];
+    o.err = (int32_t)err;
+    o.pad = 0;
+    p.out[ctl->qi] = o;
+    ctl->qi++;
+    if (ctl->qi < ctl->nq) {
+      ctl->src = p.qs[ctl->qi];
+      ctl->dst = p.qt[ctl->qi];
+      ctl->phase = kPhaseReset;
+    } else {
+      ctl->phase = kPhaseIdle;
+    }
+    return;
+  }
+  for (int c = 0; c < 2; ++c) ctl->ring[c][(ctl->lvl[c] + 1) % 3] = Ring{0u, 0u, 0ull, 0ull};
+  decide(ctl, p.mlp, p.V, p.E);
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
+  VisitCtl* ctl = p.ctl;
+  const int phase = ctl->phase;
+  if (phase == kPhaseIdle) return;
   const uint32_t nb = gridDim.x;
   const int gwave = (int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   const int nwaves = (int)(nb * kWavesPerBlock);
   const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t nthr = (int64_t)nb * kBlock;
   const uint32_t nwords = (((uint32_t)p.V + 63u) >> 6) * 2u;
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  const bool st = p.mode != kModeBfs;
-  VisitCtl* ctl = p.ctl;
-  __shared__ int s_choice[3];
+  const bool st = ctl->mode != kModeBfs;
+  VisitParams<OffT> q = p;
+  if (!ctl->record_parents) q.parent = nullptr;
+  __shared__ int s_last;
 
-  int lvl[2] = {0, 0};
-  if (!p.resume) {
-    for (int64_t v = gtid; v < p.V; v += nthr) {
+  Decision d = ctl->dec;
+  if (phase == kPhaseReset) {
+    const int32_t src = ctl->src, dst = st ? ctl->dst : -1;
+    for (int64_t v = gtid; v < q.V; v += nthr) {
       int32_t s = -1;
-      if (v == p.src)
+      if (v == src)
         s = 0;
-      else if (st && v == p.dst)
+      else if (v == dst)
         s = kColorBit;
-      p.state[v] = s;
-      if (p.parent) p.parent[v] = s != -1 ? (int32_t)v : -1;
+      q.state[v] = s;
+      if (q.parent) q.parent[v] = s != -1 ? (int32_t)v : -1;
     }
     for (int64_t w = gtid; w < nwords; w += nthr) {
-      const uint32_t bs = ((uint32_t)p.src >> 5) == (uint32_t)w ? 1u << (p.src & 31) : 0u;
-      const uint32_t bt = (st && ((uint32_t)p.dst >> 5) == (uint32_t)w) ? 1u << (p.dst & 31) : 0u;
-      p.visited[w] = bs | bt;
-      p.fbm[0][0][w] = bs;
-      p.fbm[0][1][w] = 0u;
-      p.fbm[0][2][w] = 0u;
-      if (st) {
-        p.fbm[1][0][w] = bt;
-        p.fbm[1][1][w] = 0u;
-        p.fbm[1][2][w] = 0u;
-      }
-    }
-    if (lead) {
-      for (int c = 0; c < 2; ++c)
-        for (int k = 0; k < 3; ++k) ctl->ring[c][k] = Ring{0u, 0u, 0ull, 0ull};
-      for (int c = 0; c < (st ? 2 : 1); ++c) {
-        const int32_t r = c ? p.dst : p.src;
-        const int64_t d = (int64_t)(p.off[r + 1] - p.off[r]);
-        Ring& R = ctl->ring[c][0];
-        if (d >= p.hub_threshold) {
-          p.hq[c][0][0] = r;
-          p.hb[c][0][0] = 0;
-          R.hub = (1ull << kHubEdgeBits) | (uint64_t)d;
-        } else {
-          p.q[c][0][0] = r;
-          R.n = 1u;
-        }
-        R.mf = (uint64_t)d;
-      }
-      ctl->best = ~0ull;
-      ctl->err = 0u;
+      const uint32_t bs = ((uint32_t)src >> 5) == (uint32_t)w ? 1u << (src & 31) : 0u;
+      const uint32_t bt = (dst >= 0 && ((uint32_t)dst >> 5) == (uint32_t)w) ? 1u << (dst & 31) : 0u;
+      q.visited[w] = bs | bt;
+      q.fbm[0][0][w] = bs;
+      q.fbm[0][1][w] = 0u;
+      q.fbm[0][2][w] = 0u;
+      q.fbm[1][0][w] = bt;
+      q.fbm[1][1][w] = 0u;
+      q.fbm[1][2][w] = 0u;
     }
   } else {
-    lvl[0] = p.lvl0;
-    lvl[1] = p.lvl1;
-  }
-  int status = 0;
-  if (!grid_sync(&ctl->bar, nb, p.spin_limit)) status = FC_TIMEOUT_STATUS;
-
-  // Uniform bookkeeping (identical in every block).
-  uint64_t seen_edges = 0;
-  int64_t visited_v = 0;
-  uint64_t visited_e = 0;
-  if (p.resume) {
-    seen_edges = ld_agent(&ctl->seen_edges);
-    visited_v = ld_agent(&ctl->visited_v);
-    visited_e = (uint64_t)ld_agent(&ctl->visited_e);
-  } else {
-    for (int c = 0; c < (st ? 2 : 1); ++c) {
-      const Ring* R = &ctl->ring[c][0];
-      seen_edges += ld_agent(&R->mf);
-      visited_v += ld_agent(&R->n) + (ld_agent(&R->hub) >> kHubEdgeBits);
-    }
-    visited_e = seen_edges;
-  }
-  int steps = 0, td = 0, bu = 0, depth = 0, dist = -1, meet = -1;
-  uint32_t maxf = 0;
-  int64_t expv = 0, expe = 0;
-  bool bu_mode = false;
-
-  while (status == 0) {
-    uint32_t cn[2] = {0u, 0u};
-    uint64_t cmf[2] = {0ull, 0ull}, chub[2] = {0ull, 0ull};
-    for (int c = 0; c < (st ? 2 : 1); ++c) {
-      const Ring* R = &ctl->ring[c][lvl[c] % 3];
-      cn[c] = ld_agent(&R->n);
-      cmf[c] = ld_agent(&R->mf);
-      chub[c] = ld_agent(&R->hub);
-    }
-    bool ex[2] = {false, false};
-    int strat[2] = {kTopDown, kTopDown};
-    int vw[2] = {0, 0};
-    const double mu = (double)p.E - (double)seen_edges;
-    if (p.mode == kModeBfs) {
-      ex[0] = true;
-      const uint32_t nf = cn[0] + (uint32_t)(chub[0] >> kHubEdgeBits);
-      if (!bu_mode) {
-        if ((double)cmf[0] > mu / kAlpha) bu_mode = true;
-      } else if ((double)nf * kBeta < (double)p.V) {
-        bu_mode = false;
-      }
-      strat[0] = bu_mode ? kBottomUp : kTopDown;
-    } else if (p.mode == kModePlainSt) {
-      ex[0] = ex[1] = true;
-    } else {
-      // Scheduled step: one side, chosen by the heuristic or the MLP.
-      if (threadIdx.x == 0) {
-        int side, sg, w;
-        const uint32_t nf0 = cn[0] + (uint32_t)(chub[0] >> kHubEdgeBits);
-        const uint32_t nf1 = cn[1] + (uint32_t)(chub[1] >> kHubEdgeBits);
-        side = cmf[0] <= cmf[1] ? 0 : 1;
-        sg = (double)cmf[side] > mu / kAlpha ? kBottomUp : kTopDown;
-        w = degree_width(side ? cn[1] : cn[0], cmf[side] - (chub[side] & kHubEdgeMask)) >= 16 ? 64 : 8;
-        if (p.policy == 2 && p.mlp) {
-          float f[kFeatures];
-          const uint32_t nn[2] = {nf0, nf1};
-          const uint64_t hn[2] = {chub[0] >> kHubEdgeBits, chub[1] >> kHubEdgeBits};
-          step_features(f, nn, cmf, hn, lvl[0], lvl[1], p.V, p.E, seen_edges);
-          const int a = mlp_argmin(p.mlp, f);
-          side = a / 3;
-          sg = (a % 3) == 2 ? kBottomUp : kTopDown;
-          w = (a % 3) == 1 ? 64 : 8;
-        }
-        if (p.force_side >= 0) side = p.force_side;
-        if (p.force_strategy >= 0) {
-          sg = p.force_strategy == 2 ? kBottomUp : kTopDown;
-          w = p.force_strategy == 1 ? 64 : 8;
-        }
-        if (p.trace_feat && lead) {
-          const uint32_t nn[2] = {nf0, nf1};
-          const uint64_t hn[2] = {chub[0] >> kHubEdgeBits, chub[1] >> kHubEdgeBits};
-          step_features(p.trace_feat, nn, cmf, hn, lvl[0], lvl[1], p.V, p.E, seen_edges);
-        }
-        s_choice[0] = side;
-        s_choice[1] = sg;
-        s_choice[2] = w;
-      }
-      __syncthreads();
-      const int side = s_choice[0];
-      ex[side] = true;
-      strat[side] = s_choice[1];
-      vw[side] = s_choice[2];
-      __syncthreads();
-    }
-
-    // Clear the ring slot and frontier bitmap two steps ahead of each
-    // expanding colour; nobody reads or writes them during this step.
+    if (p.force_action >= 0) apply_action(d, p.force_action);
+    const int lvl[2] = {ctl->lvl[0], ctl->lvl[1]};
     for (int c = 0; c < 2; ++c) {
-      if (!ex[c]) continue;
-      uint32_t* far = p.fbm[c][(lvl[c] + 2) % 3];
+      if (!d.ex[c]) continue;
+      uint32_t* far = q.fbm[c][(lvl[c] + 2) % 3];
       for (int64_t w = gtid; w < nwords; w += nthr) far[w] = 0u;
-      if (lead) ctl->ring[c][(lvl[c] + 2) % 3] = Ring{0u, 0u, 0ull, 0ull};
     }
-
     for (int c = 0; c < 2; ++c) {
-      if (!ex[c]) continue;
-      uint64_t mf_acc = 0;
+      if (!d.ex[c]) continue;
+      const Ring cur = ctl->ring[c][lvl[c] % 3];
       Ring* nxt = &ctl->ring[c][(lvl[c] + 1) % 3];
       const int cur_par = lvl[c] & 1, nxt_par = cur_par ^ 1;
-      uint32_t* nfbm = p.fbm[c][(lvl[c] + 1) % 3];
-      if (strat[c] == kTopDown) {
-        int w = p.vw_fixed > 0 ? p.vw_fixed : vw[c];
-        if (w == 0) w = degree_width(cn[c], cmf[c] - (chub[c] & kHubEdgeMask));
-        td_dispatch(p, w, c, lvl[c], cur_par, nxt_par, cn[c], nxt, nfbm, mf_acc, gwave, nwaves);
-        td_hubs(p, c, lvl[c], cur_par, nxt_par, chub[c], nxt, nfbm, mf_acc, gwave, nwaves);
+      uint32_t* nfbm = q.fbm[c][(lvl[c] + 1) % 3];
+      uint64_t mf_acc = 0;
+      if (d.strat[c] == kTopDown) {
+        td_dispatch(q, st, d.vw[c], c, lvl[c], cur_par, nxt_par, cur.n, nxt, nfbm, mf_acc, gwave, nwaves);
+        td_hubs(q, st, c, lvl[c], cur_par, nxt_par, cur.hub, nxt, nfbm, mf_acc, gwave, nwaves);
       } else {
-        const uint32_t* ofbm = st ? p.fbm[1 - c][lvl[1 - c] % 3] : nullptr;
-        bu_step(p, c, lvl[c], p.fbm[c][lvl[c] % 3], ofbm, nfbm, nxt_par, nxt, mf_acc, gwave, nwaves);
+        const uint32_t* ofbm = st ? q.fbm[1 - c][lvl[1 - c] % 3] : nullptr;
+        bu_step(q, st, c, lvl[c], q.fbm[c][lvl[c] % 3], ofbm, nfbm, nxt_par, nxt, mf_acc, gwave, nwaves);
       }
       const uint64_t s = wave_sum_u64(mf_acc);
       if (lane_id() == 0 && s) atomicAdd((unsigned long long*)&nxt->mf, (unsigned long long)s);
     }
-
-    if (!grid_sync(&ctl->bar, nb, p.spin_limit)) {
-      status = FC_TIMEOUT_STATUS;
-      break;
-    }
-    ++steps;
-    bool empty = false;
-    for (int c = 0; c < 2; ++c) {
-      if (!ex[c]) continue;
-      const uint32_t nf = cn[c] + (uint32_t)(chub[c] >> kHubEdgeBits);
-      expv += nf;
-      expe += (int64_t)cmf[c];
-      if (nf > maxf) maxf = nf;
-      if (strat[c] == kTopDown)
-        ++td;
-      else
-        ++bu;
-      ++lvl[c];
-      const Ring* R = &ctl->ring[c][lvl[c] % 3];
-      const uint64_t nmf = ld_agent(&R->mf);
-      const uint32_t nn = ld_agent(&R->n) + (uint32_t)(ld_agent(&R->hub) >> kHubEdgeBits);
-      seen_edges += nmf;
-      visited_e += nmf;
-      visited_v += nn;
-      if (nn == 0) empty = true;
-    }
-    if (st) {
-      const unsigned long long best = ld_agent(&ctl->best);
-      if (best != ~0ull) {
-        dist = (int)(best >> 32);
-        meet = (int)(uint32_t)best;
-        depth = steps;
-        break;
-      }
-    }
-    if (empty) break;
-    if (p.max_steps > 0 && steps >= p.max_steps) break;
-    if (steps > p.V + 2) {  // cannot happen on a valid visit; guards against a runaway loop
-      status = FC_CAPACITY_STATUS;
-      break;
-    }
   }
 
-  if (lead) {
-    const uint32_t err = ld_agent(&ctl->err);
-    VisitOut o;
-    o.status = status ? status : (err ? FC_CAPACITY_STATUS : 0);
-    o.levels = steps;
-    o.depth = depth;
-    o.dist = dist;
-    o.meet = meet;
-    o.topdown = td;
-    o.bottomup = bu;
-    o.max_frontier = (int32_t)maxf;
-    o.visited_vertices = visited_v;
-    o.visited_edges = (int64_t)visited_e;
-    o.expanded_vertices = expv;
-    o.expanded_edges = expe;
-    o.lvl_end[0] = lvl[0];
-    o.lvl_end[1] = lvl[1];
-    o.err = (int32_t)err;
-    o.pad = 0;
-    *p.out = o;
-    ctl->seen_edges = seen_edges;
-    ctl->visited_v = visited_v;
-    ctl->visited_e = (int64_t)visited_e;
-    __threadfence_system();
+  // Arrival: every wave drains its memory operations (including
+  // non-returning atomics), then thread 0 publishes the block with a release
+  // fence and takes a ticket. The block with the last ticket finalises.
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = t == nb - 1;
   }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (phase == kPhaseReset)
+    finalize_reset(q);
+  else
+    finalize_level(q, d);
+  __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 template <typename OffT>
-hipError_t launch_visit(const VisitParams<OffT>& p, int blocks, hipStream_t s) {
-  void* args[] = {(void*)&p};
-  return hipLaunchCooperativeKernel((const void*)visit_kernel<OffT>, dim3(blocks), dim3(kBlock), args, 0, s);
+hipError_t launch_phase(const VisitParams<OffT>& p, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(phase_kernel<OffT>, dim3(blocks), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
 }
 
 template <typename OffT>
-int visit_occupancy() {
+int phase_occupancy() {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, visit_kernel<OffT>, kBlock, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phase_kernel<OffT>, kBlock, 0) != hipSuccess) return 0;
   return nb;
 }
 
-template hipError_t launch_visit<int32_t>(const VisitParams<int32_t>&, int, hipStream_t);
-template hipError_t launch_visit<int64_t>(const VisitParams<int64_t>&, int, hipStream_t);
-template int visit_occupancy<int32_t>();
-template int visit_occupancy<int64_t>();
+template hipError_t launch_phase<int32_t>(const VisitParams<int32_t>&, int, hipStream_t);
+template hipError_t launch_phase<int64_t>(const VisitParams<int64_t>&, int, hipStream_t);
+template int phase_occupancy<int32_t>();
+template int phase_occupancy<int64_t>();
 
 }  // namespace fcd

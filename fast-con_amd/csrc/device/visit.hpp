@@ -1,6 +1,8 @@
-// Shared declarations of the persistent visit kernel (BFS and s–t).
+// Shared declarations of the visit engine (BFS and s–t): the control block of
+// the device-side phase machine and the launch parameters.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #include "wave.hpp"
@@ -15,12 +17,13 @@ constexpr uint64_t kHubEdgeBits = 40;         // hub counter: (entries << 40) | 
 constexpr uint64_t kHubEdgeMask = (1ull << kHubEdgeBits) - 1;
 
 enum VisitMode : int32_t { kModeBfs = 0, kModePlainSt = 1, kModeSchedSt = 2 };
-
-// Strategy of one expansion step. Top-down steps carry a virtual-warp width.
 enum Strategy : int32_t { kTopDown = 0, kBottomUp = 1 };
+enum Phase : int32_t { kPhaseIdle = 0, kPhaseReset = 1, kPhaseLevel = 2 };
 
-// Counters of one frontier (one colour, one step). Lives in a ring of 3 so a
-// slot can be cleared two steps before it is appended to.
+constexpr int32_t FC_TIMEOUT_STATUS = 8;    // fc_status FC_ERR_TIMEOUT
+constexpr int32_t FC_CAPACITY_STATUS = 6;   // fc_status FC_ERR_CAPACITY
+
+// Counters of one frontier (one colour, one step). Ring of 3 per colour.
 struct Ring {
   uint32_t n;      // normal-queue entries
   uint32_t pad;
@@ -28,24 +31,16 @@ struct Ring {
   uint64_t mf;     // sum of degrees of all entries
 };
 
-struct VisitCtl {
-  GridBar bar;
-  Ring ring[2][3];
-  unsigned long long best;   // (candidate distance << 32) | meet vertex; ~0 = none
-  uint32_t err;              // bit 0: queue slot out of range, bit 1: bad vertex id read
-  uint32_t pad;
-  // Accumulators carried across launches when a visit is resumed step by step.
-  uint64_t seen_edges;
-  int64_t visited_v;
-  int64_t visited_e;
+// What the next level launch does, per colour.
+struct Decision {
+  int32_t ex[2];      // expand this colour
+  int32_t strat[2];   // kTopDown / kBottomUp
+  int32_t vw[2];      // top-down virtual-warp width
 };
 
-constexpr int32_t FC_TIMEOUT_STATUS = 8;    // fc_status FC_ERR_TIMEOUT
-constexpr int32_t FC_CAPACITY_STATUS = 6;   // fc_status FC_ERR_CAPACITY
-
-// Written by block 0 at the end of a launch into pinned host memory.
+// Result of one query (device array, copied back by the host).
 struct VisitOut {
-  int32_t status;            // 0 ok, 8 barrier timeout, 6 capacity/corruption guard
+  int32_t status;            // 0 ok, 6 capacity/corruption guard
   int32_t levels;            // steps executed
   int32_t depth;             // s–t: steps until the meet (ST-CON DEPTH)
   int32_t dist;              // s–t distance, -1 if not connected
@@ -57,20 +52,37 @@ struct VisitOut {
   int64_t visited_edges;
   int64_t expanded_vertices;
   int64_t expanded_edges;
-  int32_t lvl_end[2];        // per-colour step count at exit (resume point)
+  int32_t lvl_end[2];
   int32_t err;
   int32_t pad;
 };
 
-// Per-step choice of the learned or heuristic scheduler (mode kModeSchedSt):
-// side 0/1, strategy, and top-down width.
-struct StepChoice {
-  int32_t side;
-  int32_t strategy;
-  int32_t vw;
+// Control block of the phase machine. Written by the finalising block of one
+// launch, read by every block of the next (kernel boundaries order them).
+struct VisitCtl {
+  int32_t phase;
+  int32_t qi;               // current query
+  int32_t nq;               // number of queries
+  int32_t mode;
+  int32_t policy;           // scheduled mode: 1 heuristic, 2 learned
+  int32_t vw_fixed;         // 0: width from frontier degree
+  int32_t record_parents;
+  int32_t max_steps;        // 0: unlimited (a traced query stops early)
+  int32_t src, dst;
+  int32_t lvl[2];
+  int32_t steps, td, bu, maxf;
+  int32_t bu_mode;
+  int32_t pad0;
+  Decision dec;
+  Ring ring[2][3];
+  unsigned long long best;  // (candidate distance << 32) | meet vertex; ~0 = none
+  uint32_t ticket;
+  uint32_t err;             // bit 0: queue slot out of range, bit 1: bad vertex id read
+  uint64_t seen_edges;      // sum of degrees of every vertex labelled so far
+  int64_t visited_v, visited_e, expv, expe;
+  float feat[12];           // features of the pending decision (scheduled mode)
 };
 
-// 12 frontier features fed to the scheduler MLP.
 constexpr int kFeatures = 12;
 constexpr int kHidden = 32;
 constexpr int kActions = 6;
@@ -84,33 +96,41 @@ struct VisitParams {
   int32_t hub_threshold;
   int64_t E;
   int32_t* state;
-  int32_t* parent;          // nullptr: not recorded
+  int32_t* parent;
   uint32_t* visited;        // ceil(V/64)*2 words
   uint32_t* fbm[2][3];      // frontier bitmaps, per colour, ring of 3
   int32_t* q[2][2];         // normal queues, per colour and parity
   int32_t* hq[2][2];        // hub-queue vertices
   int64_t* hb[2][2];        // hub-queue edge bases (ascending with slot)
-  uint32_t qcap;            // normal-queue capacity
-  uint32_t hcap;            // hub-queue capacity
+  uint32_t qcap;
+  uint32_t hcap;
   VisitCtl* ctl;
-  VisitOut* out;            // pinned host memory
-  int32_t src;
-  int32_t dst;              // -1 for BFS
-  int32_t mode;
-  int32_t policy;           // scheduled mode: 1 heuristic, 2 learned
-  int32_t vw_fixed;         // 0: width chosen from frontier degree
-  int32_t force_strategy;   // -1: scheduler decides; else forced (tracing)
-  int32_t force_side;       // -1: scheduler decides
-  int32_t max_steps;        // stop after this many steps (tracing); 0 = no limit
-  uint64_t spin_limit;      // barrier wait bound in 100 MHz ticks
-  const float* mlp;         // kMlpParams FP32 weights, or nullptr
-  // resume support for step-by-step tracing: when resume != 0 the reset
-  // phase is skipped and the step counters continue from lvl0/lvl1.
-  int32_t resume;
-  int32_t lvl0;
-  int32_t lvl1;
-  int32_t trace_slot;       // index into trace_out, or -1
-  float* trace_feat;        // kFeatures per traced step
+  const int32_t* qs;        // query sources (device), nq entries
+  const int32_t* qt;        // query targets
+  VisitOut* out;            // results (device), nq entries
+  const float* mlp;         // kMlpParams scheduler weights (device)
+  int32_t force_action;     // -1, or a scheduler action applied to this launch (tracing)
+  int32_t pad;
 };
+
+// Scheduler features of a pending step, shared by the kernel and the host tracer.
+__host__ __device__ inline void step_features(float* f, const uint32_t n[2], const uint64_t mf[2],
+                                              const uint64_t hubn[2], int lvl0, int lvl1, int64_t V, int64_t E,
+                                              uint64_t seen_edges) {
+  auto lg = [](double x) { return (float)log2(x + 1.0); };
+  const double mu = (double)E - (double)seen_edges;
+  f[0] = lg((double)n[0]) / 32.f;
+  f[1] = lg((double)mf[0]) / 40.f;
+  f[2] = lg((double)n[1]) / 32.f;
+  f[3] = lg((double)mf[1]) / 40.f;
+  f[4] = lg(mu > 0 ? mu : 0) / 40.f;
+  f[5] = lg((double)V) / 32.f;
+  f[6] = lg((double)E) / 40.f;
+  f[7] = (float)(lvl0 < 64 ? lvl0 : 64) / 64.f;
+  f[8] = (float)(lvl1 < 64 ? lvl1 : 64) / 64.f;
+  f[9] = lg(n[0] ? (double)mf[0] / n[0] : 0.0) / 20.f;
+  f[10] = lg(n[1] ? (double)mf[1] / n[1] : 0.0) / 20.f;
+  f[11] = (float)(hubn[0] + hubn[1]) / (float)(n[0] + n[1] + 1);
+}
 
 }  // namespace fcd
