@@ -58,12 +58,146 @@ __device__ __forceinline__ bool test_bit(const uint32_t* bm, int32_t v) {
   return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
-// Probe edge (v -> u) for colour c whose frontier is at level lvl. Returns
-// true when this lane claimed u for the next frontier.
+// Next-frontier appender of one wave. Normal-queue entries are staged in
+// this wave's LDS buffer; a full buffer is published by the wave with one
+// atomic, and at the end of the phase the whole block publishes its buffers,
+// degree sums and best meeting candidate with one atomic each. A single
+// frontier counter hit by every wave-iteration was the bottleneck of large
+// levels (a contended word sustains only tens of millions of atomics/s).
+constexpr uint32_t kStageCap = 1024;
+
 template <typename OffT>
-__device__ __forceinline__ bool probe_td(const VisitParams<OffT>& p, bool st, int c, int lvl, int32_t v, int32_t u,
+struct Appender {
+  const VisitParams<OffT>* p;
+  int c;
+  int nxt_par;
+  Ring* nxt;
+  uint32_t* nfbm;
+  int32_t* buf;                 // this wave's LDS stage, kStageCap entries
+  uint32_t cnt;                 // staged entries (wave-uniform)
+  uint64_t mf;                  // per-lane degree sum of claimed vertices
+  unsigned long long best;      // per-lane best meeting candidate (distance << 32 | vertex)
+};
+
+template <typename OffT>
+__device__ __forceinline__ void wave_flush(Appender<OffT>& a) {
+  if (a.cnt == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(&a.nxt->n, a.cnt);
+  base = (uint32_t)__shfl((int)base, 0);
+  int32_t* q = a.p->q[a.c][a.nxt_par];
+  for (uint32_t i = (uint32_t)lane_id(); i < a.cnt; i += kWave) {
+    const uint32_t slot = base + i;
+    if (slot < a.p->qcap)
+      q[slot] = a.buf[i];
+    else
+      atomicOr(&a.p->ctl->err, 1u);
+  }
+  __builtin_amdgcn_wave_barrier();
+  a.cnt = 0;
+}
+
+// Wave-collective: add every claimed vertex (degree deg) to the next frontier
+// of the appender's colour: normal entries through the LDS stage, hub entries
+// (degree >= hub_threshold) straight to the hub queue with their edge base.
+template <typename OffT>
+__device__ __forceinline__ void push_next(Appender<OffT>& a, bool claimed, int32_t u, int64_t deg, bool set_bitmap) {
+  const VisitParams<OffT>& p = *a.p;
+  const bool hub = claimed && deg >= p.hub_threshold;
+  const bool norm = claimed && !hub;
+  const uint64_t m = ballot(norm);
+  if (m) {
+    const uint32_t k = (uint32_t)__popcll(m);
+    if (a.cnt + k > kStageCap) wave_flush(a);
+    if (norm) a.buf[a.cnt + lane_rank(m)] = u;
+    a.cnt += k;
+  }
+  if (hub) {
+    const unsigned long long old =
+        atomicAdd((unsigned long long*)&a.nxt->hub, (1ull << kHubEdgeBits) | (unsigned long long)deg);
+    const uint32_t hs = (uint32_t)(old >> kHubEdgeBits);
+    if (hs < p.hcap) {
+      p.hq[a.c][a.nxt_par][hs] = u;
+      p.hb[a.c][a.nxt_par][hs] = (int64_t)(old & kHubEdgeMask);
+    } else {
+      atomicOr(&p.ctl->err, 1u);
+    }
+  }
+  if (claimed) {
+    a.mf += (uint64_t)deg;
+    if (set_bitmap) atomicOr(a.nfbm + (u >> 5), 1u << (u & 31));
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), o);
+    const unsigned long long w = ((unsigned long long)hi << 32) | lo;
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+// Block-collective (all threads, uniform control flow): publish every wave's
+// staged entries with one atomic, and the degree sum and best candidate.
+struct BlockShared {
+  uint32_t cnt[kWavesPerBlock];
+  unsigned long long mf[kWavesPerBlock];
+  unsigned long long best[kWavesPerBlock];
+  uint32_t base;
+};
+
+template <typename OffT>
+__device__ __forceinline__ void block_flush(Appender<OffT>& a, BlockShared& sh) {
+  const int w = (int)(threadIdx.x >> 6);
+  const unsigned long long mf = (unsigned long long)wave_sum_u64(a.mf);
+  const unsigned long long best = wave_min_u64(a.best);
+  if (lane_id() == 0) {
+    sh.cnt[w] = a.cnt;
+    sh.mf[w] = mf;
+    sh.best[w] = best;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    unsigned long long mft = 0, bt = ~0ull;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      const uint32_t c = sh.cnt[i];
+      sh.cnt[i] = tot;
+      tot += c;
+      mft += sh.mf[i];
+      bt = sh.best[i] < bt ? sh.best[i] : bt;
+    }
+    sh.base = tot ? atomicAdd(&a.nxt->n, tot) : 0u;
+    if (mft) atomicAdd((unsigned long long*)&a.nxt->mf, mft);
+    if (bt != ~0ull && bt < ld_agent(&a.p->ctl->best)) atomicMin(&a.p->ctl->best, bt);
+  }
+  __syncthreads();
+  const uint32_t base = sh.base + sh.cnt[w];
+  int32_t* q = a.p->q[a.c][a.nxt_par];
+  for (uint32_t i = (uint32_t)lane_id(); i < a.cnt; i += kWave) {
+    const uint32_t slot = base + i;
+    if (slot < a.p->qcap)
+      q[slot] = a.buf[i];
+    else
+      atomicOr(&a.p->ctl->err, 1u);
+  }
+  a.cnt = 0;
+  a.mf = 0;
+  a.best = ~0ull;
+  __syncthreads();
+}
+
+// Probe edge (v -> u) for colour c whose frontier is at level lvl. Returns
+// true when this lane claimed u for the next frontier; a meeting edge lowers
+// the lane's best candidate.
+template <typename OffT>
+__device__ __forceinline__ bool probe_td(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int32_t v, int32_t u,
                                          bool act) {
   if (!act) return false;
+  const VisitParams<OffT>& p = *a.p;
   if (!st) {
     // BFS claims through the visited bitmap: one atomicOr per newly reached
     // vertex; the plain pre-load filters most already-visited probes.
@@ -72,69 +206,41 @@ __device__ __forceinline__ bool probe_td(const VisitParams<OffT>& p, bool st, in
     if (*w & bit) return false;
     if (atomicOr(w, bit) & bit) return false;
     p.state[u] = lvl + 1;
-    if (p.parent) p.parent[u] = v;
+    if (parent) parent[u] = v;
     return true;
   }
+  const int c = a.c;
   const int32_t mine = (c ? kColorBit : 0) | (lvl + 1);
   int32_t cur = p.state[u];
   if (cur == -1) {
     cur = atomicCAS(&p.state[u], -1, mine);
     if (cur == -1) {
       atomicOr(p.visited + (u >> 5), 1u << (u & 31));
-      if (p.parent) p.parent[u] = v;
+      if (parent) parent[u] = v;
       return true;
     }
   }
   if (((cur & kColorBit) != 0) != (c != 0)) {
-    const unsigned long long cand = (unsigned long long)(lvl + 1 + (cur & kLevelMask));
-    atomicMin(&p.ctl->best, (cand << 32) | (uint32_t)u);
+    const unsigned long long cand =
+        ((unsigned long long)(lvl + 1 + (cur & kLevelMask)) << 32) | (uint32_t)u;
+    a.best = cand < a.best ? cand : a.best;
   }
   return false;
 }
 
-// Wave-collective: put every claimed vertex into the next frontier of colour c
-// (normal or hub queue by degree), optionally into the next frontier bitmap,
-// and accumulate its degree into mf_acc.
 template <typename OffT>
-__device__ __forceinline__ void append_next(const VisitParams<OffT>& p, int c, int nxt_par, Ring* nxt,
-                                            uint32_t* nfbm, bool claimed, int32_t u, uint64_t& mf_acc,
-                                            bool set_bitmap) {
-  int64_t d = 0;
-  if (claimed) d = (int64_t)(p.off[u + 1] - p.off[u]);
-  const bool hub = claimed && d >= p.hub_threshold;
-  const bool norm = claimed && !hub;
-  uint64_t m;
-  const uint32_t slot = wave_reserve(&nxt->n, norm, &m);
-  if (norm) {
-    if (slot < p.qcap)
-      p.q[c][nxt_par][slot] = u;
-    else
-      atomicOr(&p.ctl->err, 1u);
-  }
-  if (hub) {
-    const unsigned long long old =
-        atomicAdd((unsigned long long*)&nxt->hub, (1ull << kHubEdgeBits) | (unsigned long long)d);
-    const uint32_t hs = (uint32_t)(old >> kHubEdgeBits);
-    if (hs < p.hcap) {
-      p.hq[c][nxt_par][hs] = u;
-      p.hb[c][nxt_par][hs] = (int64_t)(old & kHubEdgeMask);
-    } else {
-      atomicOr(&p.ctl->err, 1u);
-    }
-  }
-  if (claimed) {
-    mf_acc += (uint64_t)d;
-    if (set_bitmap) atomicOr(nfbm + (u >> 5), 1u << (u & 31));
-  }
+__device__ __forceinline__ int64_t degree_of(const VisitParams<OffT>& p, int32_t u) {
+  return (int64_t)(p.off[u + 1] - p.off[u]);
 }
 
 template <typename OffT, int VW>
-__device__ void td_normal(const VisitParams<OffT>& p, bool st, int c, int lvl, int cur_par, int nxt_par, uint32_t n,
-                          Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
+__device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par, uint32_t n, int gwave,
+                          int nwaves) {
+  const VisitParams<OffT>& p = *a.p;
   constexpr int G = kWave / VW;
   const int lane = lane_id();
   const int grp = lane / VW, sub = lane % VW;
-  const int32_t* q = p.q[c][cur_par];
+  const int32_t* q = p.q[a.c][cur_par];
   if (n > p.qcap) n = p.qcap;
   for (uint32_t base = (uint32_t)gwave * G; base < n; base += (uint32_t)nwaves * G) {
     const uint32_t i = base + (uint32_t)grp;
@@ -149,12 +255,17 @@ __device__ void td_normal(const VisitParams<OffT>& p, bool st, int c, int lvl, i
         atomicOr(&p.ctl->err, 2u);
       }
     }
+    // two columns per lane per round: both loads are issued before either
+    // probe, halving the dependent round trips of long lists
     while (ballot(b < e)) {
-      const bool act = b < e;
-      const int32_t u = act ? p.col[b] : 0;
-      const bool cl = probe_td(p, st, c, lvl, v, u, act);
-      append_next(p, c, nxt_par, nxt, nfbm, cl, u, mf_acc, true);
-      b += VW;
+      const bool a0 = b < e, a1 = b + VW < e;
+      const int32_t u0 = a0 ? p.col[b] : 0;
+      const int32_t u1 = a1 ? p.col[b + VW] : 0;
+      const bool c0 = probe_td(a, parent, st, lvl, v, u0, a0);
+      const bool c1 = probe_td(a, parent, st, lvl, v, u1, a1);
+      push_next(a, c0, u0, c0 ? degree_of(p, u0) : 0, true);
+      push_next(a, c1, u1, c1 ? degree_of(p, u1) : 0, true);
+      b += 2 * VW;
     }
   }
 }
@@ -164,23 +275,30 @@ __device__ void td_normal(const VisitParams<OffT>& p, bool st, int c, int lvl, i
 constexpr int kHubChunk = 256;
 
 template <typename OffT>
-__device__ void td_hubs(const VisitParams<OffT>& p, bool st, int c, int lvl, int cur_par, int nxt_par,
-                        uint64_t hubpk, Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
+__device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par, uint64_t hubpk, int gwave,
+                        int nwaves) {
+  const VisitParams<OffT>& p = *a.p;
   uint32_t nh = (uint32_t)(hubpk >> kHubEdgeBits);
   if (nh > p.hcap) nh = p.hcap;
   const uint64_t H = hubpk & kHubEdgeMask;
   if (nh == 0 || H == 0) return;
   const int lane = lane_id();
-  const int32_t* hq = p.hq[c][cur_par];
-  const int64_t* hb = p.hb[c][cur_par];
+  const int32_t* hq = p.hq[a.c][cur_par];
+  const int64_t* hb = p.hb[a.c][cur_par];
   for (uint64_t cb = (uint64_t)gwave * kHubChunk; cb < H; cb += (uint64_t)nwaves * kHubChunk) {
-    uint32_t lo = 0, hi = nh;  // hb[0] == 0 <= cb; find the last slot with hb[slot] <= cb
+    // Last slot with hb[slot] <= cb (hb[0] == 0): a 64-ary search, each lane
+    // testing one of 64 evenly spaced probes per round.
+    uint32_t lo = 0, hi = nh;
     while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if ((uint64_t)hb[mid] <= cb)
-        lo = mid;
-      else
-        hi = mid;
+      const uint32_t span = hi - lo;
+      const uint32_t step = (span + 63u) / 64u;
+      const uint32_t probe = lo + (uint32_t)lane * step;
+      const bool le = probe < hi && (uint64_t)hb[probe] <= cb;
+      const uint64_t m = ballot(le);  // a prefix of lanes (hb ascending); lane 0 always set
+      const uint32_t k = 63u - (uint32_t)__builtin_clzll((unsigned long long)m);
+      lo = lo + k * step;
+      const uint32_t nhi = lo + step;
+      hi = nhi < hi ? nhi : hi;
     }
     for (int k = 0; k < kHubChunk; k += kWave) {
       const uint64_t g = cb + (uint64_t)k + (uint64_t)lane;
@@ -204,20 +322,26 @@ __device__ void td_hubs(const VisitParams<OffT>& p, bool st, int c, int lvl, int
         }
       }
       const bool ok = act && v >= 0;
-      const bool cl = probe_td(p, st, c, lvl, v, u, ok);
-      append_next(p, c, nxt_par, nxt, nfbm, cl, u, mf_acc, true);
+      const bool cl = probe_td(a, parent, st, lvl, v, u, ok);
+      push_next(a, cl, u, cl ? degree_of(p, u) : 0, true);
     }
   }
 }
+
+// Candidates of degree <= kBuLane scan their own list; longer lists are
+// scanned by the whole wave, 64 edges at a time.
+constexpr int kBuLane = 128;
 
 // Bottom-up step for colour c: every unvisited vertex looks for a neighbour in
 // c's current frontier bitmap. In s–t modes the vertices of the other colour's
 // current frontier are scanned too, so that a frontier-to-frontier edge is
 // still seen as a meeting edge.
 template <typename OffT>
-__device__ void bu_step(const VisitParams<OffT>& p, bool st, int c, int lvl, const uint32_t* cfbm,
-                        const uint32_t* ofbm, uint32_t* nfbm, int nxt_par, Ring* nxt, uint64_t& mf_acc, int gwave,
-                        int nwaves) {
+__device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool st, int lvl, const uint32_t* cfbm,
+                        const uint32_t* ofbm, int gwave, int nwaves) {
+  const VisitParams<OffT>& p = *a.p;
+  const int c = a.c;
+  uint32_t* nfbm = a.nfbm;
   const int lane = lane_id();
   const uint32_t nw64 = ((uint32_t)p.V + 63u) >> 6;
   const int32_t mine = st ? ((c ? kColorBit : 0) | (lvl + 1)) : lvl + 1;
@@ -233,23 +357,28 @@ __device__ void bu_step(const VisitParams<OffT>& p, bool st, int c, int lvl, con
       b = p.off[v];
       e = p.off[v + 1];
     }
-    const bool small = cand && (e - b) <= (OffT)32;
+    const bool small = cand && (e - b) <= (OffT)kBuLane;
     int32_t par = -1;
     if (small) {
-      // four independent column loads per round keep more requests in flight
-      for (OffT j = b; j < e && par < 0; j += 4) {
+      // eight independent column loads per round keep requests in flight;
+      // most candidates find a parent in the first round
+      for (OffT j = b; j < e && par < 0; j += 8) {
         const int32_t u0 = p.col[j];
         const int32_t u1 = j + 1 < e ? p.col[j + 1] : -1;
         const int32_t u2 = j + 2 < e ? p.col[j + 2] : -1;
         const int32_t u3 = j + 3 < e ? p.col[j + 3] : -1;
-        if (test_bit(cfbm, u0))
-          par = u0;
-        else if (u1 >= 0 && test_bit(cfbm, u1))
-          par = u1;
-        else if (u2 >= 0 && test_bit(cfbm, u2))
-          par = u2;
-        else if (u3 >= 0 && test_bit(cfbm, u3))
-          par = u3;
+        const int32_t u4 = j + 4 < e ? p.col[j + 4] : -1;
+        const int32_t u5 = j + 5 < e ? p.col[j + 5] : -1;
+        const int32_t u6 = j + 6 < e ? p.col[j + 6] : -1;
+        const int32_t u7 = j + 7 < e ? p.col[j + 7] : -1;
+        if (test_bit(cfbm, u0)) par = u0;
+        else if (u1 >= 0 && test_bit(cfbm, u1)) par = u1;
+        else if (u2 >= 0 && test_bit(cfbm, u2)) par = u2;
+        else if (u3 >= 0 && test_bit(cfbm, u3)) par = u3;
+        else if (u4 >= 0 && test_bit(cfbm, u4)) par = u4;
+        else if (u5 >= 0 && test_bit(cfbm, u5)) par = u5;
+        else if (u6 >= 0 && test_bit(cfbm, u6)) par = u6;
+        else if (u7 >= 0 && test_bit(cfbm, u7)) par = u7;
       }
     }
     uint64_t big = ballot(cand && !small);
@@ -279,11 +408,11 @@ __device__ void bu_step(const VisitParams<OffT>& p, bool st, int c, int lvl, con
     if (found && unvis) {
       claimed = true;
       p.state[v] = mine;
-      if (p.parent) p.parent[v] = par;
+      if (parent) parent[v] = par;
     } else if (found && oth) {
       const int32_t sv = p.state[v];
-      const unsigned long long cnd = (unsigned long long)(lvl + 1 + (sv & kLevelMask));
-      atomicMin(&p.ctl->best, (cnd << 32) | (uint32_t)v);
+      const unsigned long long cnd = ((unsigned long long)(lvl + 1 + (sv & kLevelMask)) << 32) | (uint32_t)v;
+      a.best = cnd < a.best ? cnd : a.best;
     }
     // Bitmap words of these 64 vertices belong to this wave during a
     // bottom-up level, so they are updated without atomics.
@@ -296,19 +425,19 @@ __device__ void bu_step(const VisitParams<OffT>& p, bool st, int c, int lvl, con
       p.visited[2u * w + 1u] |= (uint32_t)(cm >> 32);
       nfbm[2u * w + 1u] = (uint32_t)(cm >> 32);
     }
-    append_next(p, c, nxt_par, nxt, nfbm, claimed, v, mf_acc, false);
+    push_next(a, claimed, v, claimed ? (int64_t)(e - b) : 0, false);
   }
 }
 
 template <typename OffT>
-__device__ void td_dispatch(const VisitParams<OffT>& p, bool st, int vw, int c, int lvl, int cur_par, int nxt_par,
-                            uint32_t n, Ring* nxt, uint32_t* nfbm, uint64_t& mf_acc, int gwave, int nwaves) {
+__device__ __forceinline__ void td_dispatch(Appender<OffT>& a, int32_t* parent, bool st, int vw, int lvl, int cur_par, uint32_t n,
+                            int gwave, int nwaves) {
   switch (vw) {
-    case 4: td_normal<OffT, 4>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    case 8: td_normal<OffT, 8>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    case 16: td_normal<OffT, 16>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    case 32: td_normal<OffT, 32>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
-    default: td_normal<OffT, 64>(p, st, c, lvl, cur_par, nxt_par, n, nxt, nfbm, mf_acc, gwave, nwaves); break;
+    case 4: td_normal<OffT, 4>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
+    case 8: td_normal<OffT, 8>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
+    case 16: td_normal<OffT, 16>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
+    case 32: td_normal<OffT, 32>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
+    default: td_normal<OffT, 64>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
   }
 }
 
@@ -364,12 +493,14 @@ constexpr double kBeta = 24.0;
 // width 64, k=2 bottom-up.
 __device__ __forceinline__ void apply_action(Decision& d, int a) {
   const int side = a / 3, k = a % 3;
+  const int sg = k == 2 ? kBottomUp : kTopDown;
+  const int w = k == 1 ? 64 : 8;
   d.ex[0] = side == 0;
   d.ex[1] = side == 1;
-  d.strat[side] = k == 2 ? kBottomUp : kTopDown;
-  d.strat[1 - side] = kTopDown;
-  d.vw[side] = k == 1 ? 64 : 8;
-  d.vw[1 - side] = 8;
+  d.strat[0] = side == 0 ? sg : kTopDown;
+  d.strat[1] = side == 1 ? sg : kTopDown;
+  d.vw[0] = side == 0 ? w : 8;
+  d.vw[1] = side == 1 ? w : 8;
 }
 
 // Decision for the next level launch (one thread of the finalising block).
@@ -528,79 +659,89 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
   const int64_t nthr = (int64_t)nb * kBlock;
   const uint32_t nwords = (((uint32_t)p.V + 63u) >> 6) * 2u;
   const bool st = ctl->mode != kModeBfs;
-  VisitParams<OffT> q = p;
-  if (!ctl->record_parents) q.parent = nullptr;
+  int32_t* const parent = ctl->record_parents ? p.parent : nullptr;
   __shared__ int s_last;
+  __shared__ int32_t s_stage[kWavesPerBlock * kStageCap];
+  __shared__ BlockShared s_sh;
 
   Decision d = ctl->dec;
   if (phase == kPhaseReset) {
     const int32_t src = ctl->src, dst = st ? ctl->dst : -1;
-    for (int64_t v = gtid; v < q.V; v += nthr) {
+    for (int64_t v = gtid; v < p.V; v += nthr) {
       int32_t s = -1;
       if (v == src)
         s = 0;
       else if (v == dst)
         s = kColorBit;
-      q.state[v] = s;
-      if (q.parent) q.parent[v] = s != -1 ? (int32_t)v : -1;
+      p.state[v] = s;
+      if (parent) parent[v] = s != -1 ? (int32_t)v : -1;
     }
     for (int64_t w = gtid; w < nwords; w += nthr) {
       const uint32_t bs = ((uint32_t)src >> 5) == (uint32_t)w ? 1u << (src & 31) : 0u;
       const uint32_t bt = (dst >= 0 && ((uint32_t)dst >> 5) == (uint32_t)w) ? 1u << (dst & 31) : 0u;
-      q.visited[w] = bs | bt;
-      q.fbm[0][0][w] = bs;
-      q.fbm[0][1][w] = 0u;
-      q.fbm[0][2][w] = 0u;
-      q.fbm[1][0][w] = bt;
-      q.fbm[1][1][w] = 0u;
-      q.fbm[1][2][w] = 0u;
+      p.visited[w] = bs | bt;
+      p.fbm[0][0][w] = bs;
+      p.fbm[0][1][w] = 0u;
+      p.fbm[0][2][w] = 0u;
+      p.fbm[1][0][w] = bt;
+      p.fbm[1][1][w] = 0u;
+      p.fbm[1][2][w] = 0u;
     }
   } else {
     if (p.force_action >= 0) apply_action(d, p.force_action);
     const int lvl[2] = {ctl->lvl[0], ctl->lvl[1]};
+#pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (!d.ex[c]) continue;
-      uint32_t* far = q.fbm[c][(lvl[c] + 2) % 3];
+      uint32_t* far = p.fbm[c][(lvl[c] + 2) % 3];
       for (int64_t w = gtid; w < nwords; w += nthr) far[w] = 0u;
     }
+#pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (!d.ex[c]) continue;
       const Ring cur = ctl->ring[c][lvl[c] % 3];
-      Ring* nxt = &ctl->ring[c][(lvl[c] + 1) % 3];
-      const int cur_par = lvl[c] & 1, nxt_par = cur_par ^ 1;
-      uint32_t* nfbm = q.fbm[c][(lvl[c] + 1) % 3];
-      uint64_t mf_acc = 0;
+      const int cur_par = lvl[c] & 1;
+      Appender<OffT> a;
+      a.p = &p;
+      a.c = c;
+      a.nxt_par = cur_par ^ 1;
+      a.nxt = &ctl->ring[c][(lvl[c] + 1) % 3];
+      a.nfbm = p.fbm[c][(lvl[c] + 1) % 3];
+      a.buf = s_stage + (threadIdx.x >> 6) * kStageCap;
+      a.cnt = 0;
+      a.mf = 0;
+      a.best = ~0ull;
       if (d.strat[c] == kTopDown) {
-        td_dispatch(q, st, d.vw[c], c, lvl[c], cur_par, nxt_par, cur.n, nxt, nfbm, mf_acc, gwave, nwaves);
-        td_hubs(q, st, c, lvl[c], cur_par, nxt_par, cur.hub, nxt, nfbm, mf_acc, gwave, nwaves);
+        td_dispatch(a, parent, st, d.vw[c], lvl[c], cur_par, cur.n, gwave, nwaves);
+        td_hubs(a, parent, st, lvl[c], cur_par, cur.hub, gwave, nwaves);
       } else {
-        const uint32_t* ofbm = st ? q.fbm[1 - c][lvl[1 - c] % 3] : nullptr;
-        bu_step(q, st, c, lvl[c], q.fbm[c][lvl[c] % 3], ofbm, nfbm, nxt_par, nxt, mf_acc, gwave, nwaves);
+        const uint32_t* ofbm = st ? p.fbm[1 - c][lvl[1 - c] % 3] : nullptr;
+        bu_step(a, parent, st, lvl[c], p.fbm[c][lvl[c] % 3], ofbm, gwave, nwaves);
       }
-      const uint64_t s = wave_sum_u64(mf_acc);
-      if (lane_id() == 0 && s) atomicAdd((unsigned long long*)&nxt->mf, (unsigned long long)s);
+      block_flush(a, s_sh);
     }
   }
 
-  // Arrival: every wave drains its memory operations (including
-  // non-returning atomics), then thread 0 publishes the block with a release
-  // fence and takes a ticket. The block with the last ticket finalises.
+  // Arrival: every wave drains its own memory operations (including
+  // non-returning atomics), then thread 0 takes a ticket; the block with the
+  // last ticket finalises. The finaliser reads only counters that other
+  // blocks updated with device-scope atomics (ring, best, err), all
+  // performed before their tickets, so no cache writeback is needed here;
+  // the plain stores of this launch (queues, states, bitmaps) are consumed
+  // by the next launch, after the kernel boundary.
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = t == nb - 1;
   }
   __syncthreads();
   if (!s_last || threadIdx.x != 0) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   if (phase == kPhaseReset)
-    finalize_reset(q);
+    finalize_reset(p);
   else
-    finalize_level(q, d);
+    finalize_level(p, d);
   __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 template <typename OffT>
