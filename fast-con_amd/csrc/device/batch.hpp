@@ -1,0 +1,93 @@
+// Batched bit-parallel s–t engine: declarations shared by the kernels and
+// the host driver.
+#pragma once
+
+#include <stdint.h>
+
+#include "visit.hpp"
+
+namespace fcd {
+
+enum BatchPhase : int32_t {
+  kBIdle = 0,
+  kBInit = 1,      // seed the wave's 64 queries
+  kBExpand = 2,    // top-down: frontier vertices OR their masks into neighbours
+  kBCommit = 3,    // new masks -> visited, meets, next frontier lists
+  kBBottomUp = 4,  // bottom-up step of one side (expand + commit in one pass)
+  kBClearF = 5,    // clear the consumed frontier masks after a bottom-up step
+  kBCleanup = 6    // zero every mask the wave touched, write results
+};
+
+enum BatchSides : int32_t { kSideS = 1, kSideT = 2, kSideBoth = 3 };
+
+struct BatchDecision {
+  int32_t sides;     // kSideS / kSideT / kSideBoth
+  int32_t strat;     // kTopDown / kBottomUp (bottom-up: one side only)
+  int32_t vw;        // top-down virtual-warp width
+  int32_t action;    // scheduler action (0..5), -1 for the plain both-sides step
+};
+
+struct BatchCtl {
+  int32_t phase;
+  int32_t wave;            // current wave index
+  int32_t nwaves;
+  int32_t nq;              // total queries
+  int32_t policy;
+  int32_t vw_fixed;
+  int32_t tile;            // edges per hub chunk
+  int32_t lvl[2];          // levels done by each side in this wave
+  int32_t steps;
+  int32_t cur[2];          // parity of the current frontier list/mask per side
+  int32_t bu_side;         // side of the last bottom-up step (for kBClearF)
+  BatchDecision dec;
+  unsigned long long act;  // unresolved queries of the wave
+  uint32_t fl_n[2][2];     // normal frontier list sizes [side][parity]
+  unsigned long long hub[2][2];   // hub lists: (entries << 40) | edges
+  unsigned long long fl_mf[2][2]; // degree sums of the frontier (normal + hub)
+  uint32_t touched_n;
+  uint32_t dirty_n;
+  unsigned long long odd, even, any_s, any_t;   // block-reduced results of a commit
+  uint32_t ticket;
+  uint32_t err;
+  // statistics (whole batch)
+  int64_t levels_total;
+  int64_t active_vertices;  // sum over steps of frontier vertices processed
+  int64_t active_edges;     // sum over steps of their degrees
+  int64_t touched_total;
+  float feat[12];
+};
+
+template <typename OffT>
+struct BatchParams {
+  const OffT* off;
+  const int32_t* col;
+  int32_t V;
+  int32_t hub_threshold;
+  int64_t E;
+  // Per-vertex record of four 64-bit words, rec[4u + 2x] = side-x visited
+  // mask and rec[4u + 2x + 1] = side-x masks ORed in during the current
+  // top-down step: one 16-byte load gives a probe both words it needs.
+  unsigned long long* rec;     // [4*V]
+  unsigned long long* F[2][2]; // frontier masks [side][parity], V entries each
+  uint32_t* tb;                // touched bitmap, ceil(V/32) words
+  int32_t* fl[2][2];           // frontier lists [side][parity], V entries each
+  int32_t* hl[2][2];           // hub frontier lists (degree >= hub_threshold)
+  int64_t* hb[2][2];           // their edge bases (ascending with slot)
+  uint32_t hcap;
+  int32_t* touched;            // vertices whose N word became non-zero this step
+  int32_t* dirty;              // vertices whose visited words became non-zero this wave
+  BatchCtl* ctl;
+  const int32_t* qs;           // queries (device)
+  const int32_t* qt;
+  int32_t* dist;               // results (device), nq entries
+  const float* mlp;
+  int32_t force_action;
+  int32_t pad;
+};
+
+template <typename OffT>
+hipError_t launch_batch_phase(const BatchParams<OffT>& p, int blocks, hipStream_t s);
+template <typename OffT>
+int batch_occupancy();
+
+}  // namespace fcd

@@ -313,10 +313,10 @@ int orc_find_stcon_roots(int n, int64_t nglobalverts, const int64_t* off, const 
 
 /* ---------------------------------------------------------------------- */
 /* Batched bit-parallel s–t, restated with per-query bit sets: wave of up   */
-/* to 64 queries, both sides advanced one level per iteration. For every    */
-/* iteration it reports A (vertices with a non-zero frontier word on either */
-/* side) and their degree sum, and returns per-query distances. This is the */
-/* work model of the batched engine's plain schedule.                       */
+/* to 64 queries, both sides advanced one level per iteration. Summed over  */
+/* iterations and sides it reports A (frontier vertices whose word has a    */
+/* live query) and their degree sum, and returns per-query distances. This  */
+/* is the work model of the batched engine's plain schedule.                */
 /* ---------------------------------------------------------------------- */
 int orc_batch_wave(int64_t V, const int64_t* off, const int32_t* col, int nq, const int32_t* s, const int32_t* t,
                    int32_t* dist, int64_t* a_sum, int64_t* e_sum, int32_t* iters) {
@@ -344,8 +344,10 @@ int orc_batch_wave(int64_t V, const int64_t* off, const int32_t* col, int nq, co
     for (int64_t v = 0; v < V; ++v) {
       uint64_t a = (fs[v] | ft[v]) & active;
       if (!a) continue;
-      A++;
-      Es += off[v + 1] - off[v];
+      /* work counted per side: a vertex in both frontiers is expanded twice */
+      int sides = ((fs[v] & active) != 0) + ((ft[v] & active) != 0);
+      A += sides;
+      Es += sides * (off[v + 1] - off[v]);
       for (int64_t e = off[v]; e < off[v + 1]; ++e) {
         int32_t u = col[e];
         ns[u] |= fs[v] & active;
