@@ -18,7 +18,7 @@
 //
 // Execution follows the visit engine: one launch per phase, the last block
 // to finish decides the next phase, the host replays captured launches.
-// Only the masks a wave touched are cleared afterwards (dirty list), so a
+// Only the masks a wave touched are cleared afterwards (dirty bitmap), so a
 // wave costs nothing proportional to V beyond the bottom-up steps.
 #include "batch.hpp"
 
@@ -26,10 +26,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "engine.hpp"
+#include "hubstage.hpp"
 #include "mlp.hpp"
 
 namespace fcd {
@@ -37,7 +40,8 @@ namespace fcd {
 using fc::fail;
 
 constexpr uint32_t kBStage = 512;  // staged entries per wave and list
-constexpr int kBLists = 3;
+constexpr uint64_t kFilterMinVertices = 4096;   // list size worth a filter launch
+constexpr uint64_t kFilterMinEdges = 1u << 16;  // or frontier degree sum
 
 // ---------------------------------------------------------------------------
 // LDS-staged appends (one wave stage per list; block publishes with one atomic)
@@ -75,15 +79,17 @@ __device__ __forceinline__ void stage_push(Stage& s, bool want, int32_t v, uint3
 }
 
 struct BShared {
-  uint32_t cnt[kBLists][kWavesPerBlock];
-  uint32_t base[kBLists];
-  unsigned long long red[8][kWavesPerBlock];
+  uint32_t cnt[2][kWavesPerBlock];
+  uint32_t base[2];
+  unsigned long long red[kRCount][kWavesPerBlock];
+  HubShared hub[2];
   int last;
 };
 
-// Block-collective publication of the staged lists (all threads call).
-__device__ __forceinline__ void stage_block_part(Stage& st, int l, uint32_t* counter, int32_t* list, uint32_t cap,
-                                                 uint32_t* err, BShared& sh) {
+// Block-collective publication of the two staged frontier lists (all threads
+// call, uniform control flow).
+__device__ __forceinline__ void stage_block_part(Stage& st, int l, int32_t* list, uint32_t cap, uint32_t* err,
+                                                 BShared& sh) {
   const int w = (int)(threadIdx.x >> 6);
   const uint32_t base = sh.base[l] + sh.cnt[l][w];
   for (uint32_t i = (uint32_t)lane_id(); i < st.cnt; i += kWave) {
@@ -96,17 +102,15 @@ __device__ __forceinline__ void stage_block_part(Stage& st, int l, uint32_t* cou
   st.cnt = 0;
 }
 
-__device__ __forceinline__ void stages_block_flush(Stage& s0, Stage& s1, Stage& s2, int nl, uint32_t* const* counters,
-                                                   int32_t* const* lists, const uint32_t* caps, uint32_t* err,
-                                                   BShared& sh) {
+__device__ __forceinline__ void stages_block_flush(Stage& s0, Stage& s1, uint32_t* c0, uint32_t* c1, int32_t* l0,
+                                                   int32_t* l1, uint32_t cap, uint32_t* err, BShared& sh) {
   const int w = (int)(threadIdx.x >> 6);
   if (lane_id() == 0) {
     sh.cnt[0][w] = s0.cnt;
     sh.cnt[1][w] = s1.cnt;
-    sh.cnt[2][w] = s2.cnt;
   }
   __syncthreads();
-  if ((int)threadIdx.x < nl) {
+  if (threadIdx.x < 2) {
     const int l = (int)threadIdx.x;
     uint32_t tot = 0;
     for (int i = 0; i < kWavesPerBlock; ++i) {
@@ -114,40 +118,11 @@ __device__ __forceinline__ void stages_block_flush(Stage& s0, Stage& s1, Stage& 
       sh.cnt[l][i] = tot;
       tot += c;
     }
-    sh.base[l] = tot ? atomicAdd(counters[l], tot) : 0u;
+    sh.base[l] = tot ? atomicAdd(l ? c1 : c0, tot) : 0u;
   }
   __syncthreads();
-  stage_block_part(s0, 0, counters[0], lists[0], caps[0], err, sh);
-  if (nl > 1) stage_block_part(s1, 1, counters[1], lists[1], caps[1], err, sh);
-  if (nl > 2) stage_block_part(s2, 2, counters[2], lists[2], caps[2], err, sh);
-  __syncthreads();
-}
-
-// Block reduction of up to 8 per-lane 64-bit values; op[k] = 0 sum, 1 or.
-// Thread 0 then applies them to dst[k] with one atomic each (skipping zeros).
-__device__ __forceinline__ void block_reduce_apply(unsigned long long* vals, const int* op, int nv, unsigned long long* const* dst,
-                                   BShared& sh) {
-  const int w = (int)(threadIdx.x >> 6);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (k >= nv) break;
-    unsigned long long v = vals[k];
-    v = op[k] ? (unsigned long long)wave_or_u64(v) : (unsigned long long)wave_sum_u64(v);
-    if (lane_id() == 0) sh.red[k][w] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < nv; ++k) {
-      unsigned long long a = 0;
-      for (int i = 0; i < kWavesPerBlock; ++i) a = op[k] ? (a | sh.red[k][i]) : (a + sh.red[k][i]);
-      if (a) {
-        if (op[k])
-          atomicOr(dst[k], a);
-        else
-          atomicAdd(dst[k], a);
-      }
-    }
-  }
+  stage_block_part(s0, 0, l0, cap, err, sh);
+  stage_block_part(s1, 1, l1, cap, err, sh);
   __syncthreads();
 }
 
@@ -156,51 +131,112 @@ __device__ __forceinline__ int64_t bdeg(const BatchParams<OffT>& p, int32_t u) {
   return (int64_t)(p.off[u + 1] - p.off[u]);
 }
 
-// Put a newly reached vertex u of side x into the next frontier: the normal
-// list through the stage, or the hub list (with its edge base) by degree.
-template <typename OffT>
-__device__ __forceinline__ void push_frontier(const BatchParams<OffT>& p, Stage& st, int x, int nxt, bool want,
-                                              int32_t u, int64_t deg) {
-  const bool hub = want && deg >= p.hub_threshold;
-  stage_push(st, want && !hub, u, &p.ctl->fl_n[x][nxt], p.fl[x][nxt], (uint32_t)p.V, &p.ctl->err);
-  if (hub) {
-    const unsigned long long old =
-        atomicAdd(&p.ctl->hub[x][nxt], (1ull << kHubEdgeBits) | (unsigned long long)deg);
-    const uint32_t hs = (uint32_t)(old >> kHubEdgeBits);
-    if (hs < p.hcap) {
-      p.hl[x][nxt][hs] = u;
-      p.hb[x][nxt][hs] = (int64_t)(old & kHubEdgeMask);
-    } else {
-      atomicOr(&p.ctl->err, 1u);
+// Block reduction of the kRCount launch values (op: OR for the first four,
+// sum for the rest) into this block's slot; lanes 0..kRCount-1 of wave 0
+// issue one atomic each, skipping zeros.
+__device__ __forceinline__ void block_reduce_slots(const unsigned long long* vals, unsigned long long* red,
+                                                   BShared& sh) {
+  const int w = (int)(threadIdx.x >> 6);
+#pragma unroll
+  for (int k = 0; k < kRCount; ++k) {
+    const unsigned long long v = k < kRMfS ? (unsigned long long)wave_or_u64(vals[k])
+                                           : (unsigned long long)wave_sum_u64(vals[k]);
+    if (lane_id() == 0) sh.red[k][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kRCount) {
+    const int k = (int)threadIdx.x;
+    unsigned long long a = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) a = k < kRMfS ? (a | sh.red[k][i]) : (a + sh.red[k][i]);
+    unsigned long long* dst = red + (size_t)(blockIdx.x % kRedSlots) * kRedWords + k;
+    if (a) {
+      if (k < kRMfS)
+        atomicOr(dst, a);
+      else
+        atomicAdd(dst, a);
     }
   }
+  __syncthreads();
 }
 
-// Top-down probe of edge (v -> u) for side x with frontier mask fm.
-template <typename OffT>
-__device__ __forceinline__ void probe(const BatchParams<OffT>& p, Stage& tst, int x, unsigned long long fm, int32_t u,
-                                      bool act) {
-  bool touched = false;
-  if (act && fm) {
-    unsigned long long* r = p.rec + 4 * (int64_t)u + 2 * x;
-    const ulonglong2 vn = *reinterpret_cast<const ulonglong2*>(r);  // {visited, next}
-    const unsigned long long m = fm & ~vn.x;
-    // skip the atomic when every new bit is already in the next word
-    if (m & ~vn.y) {
-      const unsigned long long old = atomicOr(r + 1, m);
-      if (old == 0) {
-        const uint32_t bit = 1u << (u & 31);
-        touched = !(atomicOr(&p.tb[u >> 5], bit) & bit);
-      }
+struct RedTot {
+  unsigned long long v[kRCount];
+};
+
+// Wave 0 of the finalising block: sum the slots and clear them.
+__device__ __forceinline__ RedTot collect_slots(unsigned long long* red) {
+  const int lane = lane_id();
+  RedTot t;
+#pragma unroll
+  for (int k = 0; k < kRCount; ++k) {
+    unsigned long long v = 0;
+    if (lane < kRedSlots) {
+      unsigned long long* a = red + (size_t)lane * kRedWords + k;
+      v = ld_agent(a);
+      if (v) *a = 0ull;
     }
+    t.v[k] = k < kRMfS ? (unsigned long long)wave_or_u64(v) : (unsigned long long)wave_sum_u64(v);
   }
-  stage_push(tst, touched, u, &p.ctl->touched_n, p.touched, (uint32_t)p.V, &p.ctl->err);
+  return t;
+}
+
+// Per-wave append state of the commit, bottom-up and filter phases: the next
+// frontier lists of both sides (normal vertices and hubs).
+struct Appends {
+  Stage s[2];
+  HubStage h[2];
+};
+
+// Put a newly reached vertex u of side x into the next frontier: the normal
+// list or, by degree, the hub list (whose edge bases the hub stage assigns).
+template <typename OffT>
+__device__ __forceinline__ void push_frontier(const BatchParams<OffT>& p, Appends& ap, int x, int nxt, bool want,
+                                              int32_t u, int64_t deg) {
+  const bool hub = want && deg >= p.hub_threshold;
+  stage_push(ap.s[x], want && !hub, u, &p.ctl->fl_n[x][nxt], p.fl[x][nxt], (uint32_t)p.V, &p.ctl->err);
+  hub_push(ap.h[x], hub, u, deg, &p.ctl->hub[x][nxt], p.hl[x][nxt], p.hb[x][nxt], p.hcap, &p.ctl->err);
+}
+
+template <typename OffT>
+__device__ __forceinline__ void appends_block_flush(const BatchParams<OffT>& p, Appends& ap, const int* nxt,
+                                                    BShared& sh) {
+  BatchCtl* ctl = p.ctl;
+  stages_block_flush(ap.s[0], ap.s[1], &ctl->fl_n[0][nxt[0]], &ctl->fl_n[1][nxt[1]], p.fl[0][nxt[0]],
+                     p.fl[1][nxt[1]], (uint32_t)p.V, &ctl->err, sh);
+  hub_block_flush(ap.h[0], &ctl->hub[0][nxt[0]], p.hl[0][nxt[0]], p.hb[0][nxt[0]], p.hcap, &ctl->err, sh.hub[0]);
+  hub_block_flush(ap.h[1], &ctl->hub[1][nxt[1]], p.hl[1][nxt[1]], p.hb[1][nxt[1]], p.hcap, &ctl->err, sh.hub[1]);
+}
+
+// Top-down probe of edge (v -> u) for side x with frontier mask fm, split in
+// a load and an update so that a lane's loads for several edges are in
+// flight together. The update ORs the new bits into u's N word with an
+// atomic that returns nothing and, when the loaded N word was zero, marks
+// u's group of 32 vertices with a plain byte store (every writer stores the
+// same value, so the race is benign). Every vertex whose N word becomes
+// non-zero is in a marked group: the first atomic on it is issued by a lane
+// whose earlier load saw zero.
+template <typename OffT>
+__device__ __forceinline__ ulonglong2 probe_load(const BatchParams<OffT>& p, int x, unsigned long long fm, int32_t u,
+                                                 bool act) {
+  ulonglong2 vn = make_ulonglong2(~0ull, 0ull);
+  if (act && fm) vn = *reinterpret_cast<const ulonglong2*>(p.rec + 4 * (int64_t)u + 2 * x);  // {visited, next}
+  return vn;
+}
+
+template <typename OffT>
+__device__ __forceinline__ void probe_apply(const BatchParams<OffT>& p, int x, unsigned long long fm, int32_t u,
+                                            ulonglong2 vn) {
+  const unsigned long long m = fm & ~vn.x;
+  if (m & ~vn.y) {
+    __hip_atomic_fetch_or(p.rec + 4 * (int64_t)u + 2 * x + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (vn.y == 0) p.tm[u >> 5] = 1;
+  }
 }
 
 template <typename OffT, int VW>
-__device__ __forceinline__ void expand_normal(const BatchParams<OffT>& p, Stage& tst, int x, int par, uint32_t n,
-                              unsigned long long act, int gwave, int nwaves, unsigned long long& a_cnt,
-                              unsigned long long& e_cnt) {
+__device__ __forceinline__ void expand_normal(const BatchParams<OffT>& p, int x, int par, uint32_t n,
+                                              unsigned long long act, int gwave, int nwaves,
+                                              unsigned long long& a_cnt, unsigned long long& e_cnt) {
   constexpr int G = kWave / VW;
   const int lane = lane_id();
   const int grp = lane / VW, sub = lane % VW;
@@ -235,17 +271,23 @@ __device__ __forceinline__ void expand_normal(const BatchParams<OffT>& p, Stage&
       const bool a0 = b < e, a1 = b + VW < e;
       const int32_t u0 = a0 ? p.col[b] : 0;
       const int32_t u1 = a1 ? p.col[b + VW] : 0;
-      probe(p, tst, x, fm, u0, a0);
-      probe(p, tst, x, fm, u1, a1);
+      const ulonglong2 r0 = probe_load(p, x, fm, u0, a0);
+      const ulonglong2 r1 = probe_load(p, x, fm, u1, a1);
+      if (a0) probe_apply(p, x, fm, u0, r0);
+      if (a1) probe_apply(p, x, fm, u1, r1);
       b += 2 * VW;
     }
   }
 }
 
+// Hub frontier vertices are expanded edge-parallel: the concatenated
+// adjacency of the hub list is cut into chunks of `chunk` edges, one chunk per
+// wave at a time, four edges per lane and pass.
 template <typename OffT>
-__device__ __forceinline__ void expand_hubs(const BatchParams<OffT>& p, Stage& tst, int x, int par, unsigned long long hubpk,
-                            unsigned long long act, int chunk, int gwave, int nwaves, unsigned long long& a_cnt,
-                            unsigned long long& e_cnt) {
+__device__ __forceinline__ void expand_hubs(const BatchParams<OffT>& p, int x, int par, unsigned long long hubpk,
+                                            unsigned long long act, int chunk, int gwave, int nwaves,
+                                            unsigned long long& a_cnt, unsigned long long& e_cnt) {
+  constexpr int R = 4;
   uint32_t nh = (uint32_t)(hubpk >> kHubEdgeBits);
   if (nh > p.hcap) nh = p.hcap;
   const uint64_t H = hubpk & kHubEdgeMask;
@@ -255,6 +297,7 @@ __device__ __forceinline__ void expand_hubs(const BatchParams<OffT>& p, Stage& t
   const int64_t* hb = p.hb[x][par];
   const unsigned long long* F = p.F[x][par];
   for (uint64_t cb = (uint64_t)gwave * (uint64_t)chunk; cb < H; cb += (uint64_t)nwaves * (uint64_t)chunk) {
+    // 64-ary search: the last hub whose edge base is <= cb
     uint32_t lo = 0, hi = nh;
     while (hi - lo > 1) {
       const uint32_t span = hi - lo;
@@ -267,162 +310,283 @@ __device__ __forceinline__ void expand_hubs(const BatchParams<OffT>& p, Stage& t
       const uint32_t nhi = lo + step;
       hi = nhi < hi ? nhi : hi;
     }
-    for (int k = 0; k < chunk; k += kWave) {
-      const uint64_t g = cb + (uint64_t)k + (uint64_t)lane;
-      bool act_e = g < H;
-      int32_t u = 0;
-      unsigned long long fm = 0;
-      if (act_e) {
-        uint32_t s = lo;
-        while (s + 1 < nh && (uint64_t)hb[s + 1] <= g) ++s;
-        const int32_t v = hl[s];
-        bool good = (uint32_t)v < (uint32_t)p.V && (uint64_t)hb[s] <= g;
-        OffT k2 = 0;
-        if (good) {
-          k2 = p.off[v] + (OffT)(g - (uint64_t)hb[s]);
-          good = k2 < p.off[v + 1];
-        }
-        if (good) {
-          u = p.col[k2];
-          fm = F[v] & act;
-          if (fm) {
-            e_cnt += 1;
-            if ((uint64_t)hb[s] == g) a_cnt += 1;  // first edge of a hub with live queries
+    uint32_t s = lo;
+    for (int k = 0; k < chunk; k += R * kWave) {
+      OffT pos[R];
+      unsigned long long fm[R];
+      bool ae[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint64_t g = cb + (uint64_t)(k + j * kWave + lane);
+        ae[j] = k + j * kWave < chunk && g < H;
+        fm[j] = 0;
+        pos[j] = 0;
+        if (ae[j]) {
+          while (s + 1 < nh && (uint64_t)hb[s + 1] <= g) ++s;
+          const int32_t v = hl[s];
+          const uint64_t b0 = (uint64_t)hb[s];
+          bool good = (uint32_t)v < (uint32_t)p.V && b0 <= g;
+          if (good) {
+            pos[j] = p.off[v] + (OffT)(g - b0);
+            good = pos[j] < p.off[v + 1];
           }
-        } else {
-          atomicOr(&p.ctl->err, 2u);
-          act_e = false;
+          if (good) {
+            fm[j] = F[v] & act;
+            if (fm[j]) {
+              e_cnt += 1;
+              if (b0 == g) a_cnt += 1;  // first edge of a hub with live queries
+            }
+          } else {
+            atomicOr(&p.ctl->err, 2u);
+            ae[j] = false;
+          }
         }
       }
-      probe(p, tst, x, fm, u, act_e);
+      int32_t u[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) u[j] = (ae[j] && fm[j]) ? p.col[pos[j]] : 0;
+      ulonglong2 r[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) r[j] = probe_load(p, x, fm[j], u[j], ae[j]);
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (ae[j] && fm[j]) probe_apply(p, x, fm[j], u[j], r[j]);
     }
   }
 }
 
 template <typename OffT>
-__device__ __forceinline__ void expand_side(const BatchParams<OffT>& p, Stage& tst, int x, int vw, int chunk, unsigned long long act,
-                            int gwave, int nwaves, unsigned long long& a_cnt, unsigned long long& e_cnt) {
+__device__ __forceinline__ void expand_side(const BatchParams<OffT>& p, int x, int vw, int chunk,
+                                            unsigned long long act, int gwave, int nwaves,
+                                            unsigned long long& a_cnt, unsigned long long& e_cnt) {
   const BatchCtl* ctl = p.ctl;
   const int par = ctl->cur[x];
   const uint32_t n = ctl->fl_n[x][par];
   switch (vw) {
-    case 4: expand_normal<OffT, 4>(p, tst, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
-    case 8: expand_normal<OffT, 8>(p, tst, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
-    case 16: expand_normal<OffT, 16>(p, tst, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
-    case 32: expand_normal<OffT, 32>(p, tst, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
-    default: expand_normal<OffT, 64>(p, tst, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
+    case 4: expand_normal<OffT, 4>(p, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
+    case 8: expand_normal<OffT, 8>(p, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
+    case 16: expand_normal<OffT, 16>(p, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
+    case 32: expand_normal<OffT, 32>(p, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
+    default: expand_normal<OffT, 64>(p, x, par, n, act, gwave, nwaves, a_cnt, e_cnt); break;
   }
-  expand_hubs(p, tst, x, par, ctl->hub[x][par], act, chunk, gwave, nwaves, a_cnt, e_cnt);
+  expand_hubs(p, x, par, ctl->hub[x][par], act, chunk, gwave, nwaves, a_cnt, e_cnt);
 }
 
-// Commit one touched vertex (both sides); appends go through stages 0/1
-// (frontier lists of s and t) and 2 (dirty list).
-template <typename OffT>
-__device__ __forceinline__ void commit_vertex(const BatchParams<OffT>& p, Stage& s0, Stage& s1, Stage& s2, bool valid,
-                                              int32_t u,
-                                              unsigned long long act, const int* nxt, unsigned long long* red) {
-  unsigned long long nS = 0, nT = 0, vS = 0, vT = 0;
-  int64_t deg = 0;
-  if (valid) {
-    const uint32_t bit = 1u << (u & 31);
-    atomicAnd(&p.tb[u >> 5], ~bit);
-    ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
-    const ulonglong2 a = r[0], b = r[1];  // {visS, NS}, {visT, NT}
-    vS = a.x;
-    vT = b.x;
-    nS = a.y & ~vS & act;
-    nT = b.y & ~vT & act;
-    red[0] |= (nS & vT) | (nT & vS);  // odd meets
-    red[1] |= nS & nT;                 // even meets
-    red[2] |= nS;
-    red[3] |= nT;
-    if (a.y) r[0] = make_ulonglong2(vS | nS, 0ull);
-    if (b.y) r[1] = make_ulonglong2(vT | nT, 0ull);
-    if (nS) p.F[0][nxt[0]][u] = nS;
-    if (nT) p.F[1][nxt[1]][u] = nT;
-    if (nS | nT) deg = bdeg(p, u);
-    if (nS) red[4] += (unsigned long long)deg;
-    if (nT) red[5] += (unsigned long long)deg;
+// Scan of a per-32-vertex byte array (touched or dirty groups) in vertex
+// order: the wave reads 64 group bytes, then handles the non-zero groups two
+// at a time, lanes 0..31 on the first and 32..63 on the second, so that the
+// per-vertex records of a group are read and written in address order. Each
+// group belongs to one wave of the launch. fn(u, valid, g0, g1) runs with
+// every lane active; g0/g1 are the groups of the low and high half-wave.
+template <typename Fn>
+__device__ __forceinline__ void scan_groups(uint8_t* marks, int64_t V, int gwave, int nwaves, Fn&& fn) {
+  const int lane = lane_id();
+  const int64_t groups = (V + 31) / 32;
+  for (int64_t gb = (int64_t)gwave * kWave; gb < groups; gb += (int64_t)nwaves * kWave) {
+    const int64_t gi = gb + lane;
+    const bool marked = gi < groups && marks[gi] != 0;
+    if (marked) marks[gi] = 0;
+    uint64_t nz = ballot(marked);
+    while (nz) {
+      const int k0 = __builtin_ctzll(nz);
+      nz &= nz - 1;
+      int k1 = -1;
+      if (nz) {
+        k1 = __builtin_ctzll(nz);
+        nz &= nz - 1;
+      }
+      const int mine = lane < 32 ? k0 : k1;
+      const int64_t u = (gb + (mine < 0 ? 0 : mine)) * 32 + (lane & 31);
+      fn((int32_t)u, mine >= 0 && u < V, gb + k0, k1 >= 0 ? gb + k1 : (int64_t)-1);
+    }
   }
-  push_frontier(p, s0, 0, nxt[0], nS != 0, u, deg);
-  push_frontier(p, s1, 1, nxt[1], nT != 0, u, deg);
-  stage_push(s2, ((vS | vT) == 0) && ((nS | nT) != 0), u, &p.ctl->dirty_n, p.dirty, (uint32_t)p.V + 256u,
-             &p.ctl->err);
 }
 
-// Bottom-up step for side x: every vertex still needed by some query ORs the
-// current frontier masks of its neighbours, stopping once it has them all.
+// Mark dirty vertices (visited words became non-zero this wave) in the dirty
+// bitmap, for a pass whose lanes 0..31 / 32..63 hold the 32 vertices of
+// groups g0 / g1 (each group owned by this wave in this launch).
 template <typename OffT>
-__device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Stage& s0, Stage& s1, Stage& s2, int x,
-                                          unsigned long long act, int gwave, int nwaves,
-                          unsigned long long* red, unsigned long long& a_cnt, unsigned long long& e_cnt) {
+__device__ __forceinline__ void mark_dirty(const BatchParams<OffT>& p, bool dirty, int64_t g0, int64_t g1) {
+  const uint64_t dm = ballot(dirty);
+  const int lane = lane_id();
+  if (lane == 0 && (uint32_t)dm) p.db[g0] |= (uint32_t)dm;
+  if (lane == 32 && (dm >> 32) && g1 >= 0) p.db[g1] |= (uint32_t)(dm >> 32);
+}
+
+// Commit: every vertex of a touched group folds its N words into its visited
+// words, and newly reached vertices join the next frontiers.
+template <typename OffT>
+__device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends& ap, unsigned long long act,
+                                            const int* nxt, int gwave, int nwaves, unsigned long long* red) {
+  scan_groups(p.tm, p.V, gwave, nwaves, [&](int32_t u, bool valid, int64_t g0, int64_t g1) {
+    unsigned long long nS = 0, nT = 0, vS = 0, vT = 0;
+    int64_t deg = 0;
+    if (valid) {
+      ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
+      const ulonglong2 a = r[0], b = r[1];  // {visS, NS}, {visT, NT}
+      vS = a.x;
+      vT = b.x;
+      nS = a.y & ~vS & act;
+      nT = b.y & ~vT & act;
+      red[kROdd] |= (nS & vT) | (nT & vS);  // odd meets
+      red[kREven] |= nS & nT;                // even meets
+      red[kRAnyS] |= nS;
+      red[kRAnyT] |= nT;
+      if (a.y | b.y) red[kRTouched] += 1;
+      if (a.y) r[0] = make_ulonglong2(vS | nS, 0ull);
+      if (b.y) r[1] = make_ulonglong2(vT | nT, 0ull);
+      if (nS) p.F[0][nxt[0]][u] = nS;
+      if (nT) p.F[1][nxt[1]][u] = nT;
+      if (nS | nT) deg = bdeg(p, u);
+      if (nS) red[kRMfS] += (unsigned long long)deg;
+      if (nT) red[kRMfT] += (unsigned long long)deg;
+    }
+    push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
+    push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
+    mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), g0, g1);
+  });
+}
+
+// Bottom-up step for the sides in SIDES (bit x = side x): every vertex still
+// needed by some live query ORs the current frontier masks of its
+// neighbours, stopping once it has every bit it needs. Both sides share one
+// scan of the adjacency list.
+template <typename OffT, int SIDES>
+__device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& ap, unsigned long long act,
+                                          int gwave, int nwaves, unsigned long long* red) {
   const BatchCtl* ctl = p.ctl;
-  const int cur = ctl->cur[x], nxt = cur ^ 1;
-  const unsigned long long* F = p.F[x][cur];
+  constexpr bool DS = (SIDES & 1) != 0, DT = (SIDES & 2) != 0;
+  const int nxt[2] = {ctl->cur[0] ^ 1, ctl->cur[1] ^ 1};
+  const unsigned long long* FS = p.F[0][ctl->cur[0]];
+  const unsigned long long* FT = p.F[1][ctl->cur[1]];
   const int lane = lane_id();
   const int64_t nthr = (int64_t)nwaves * kWave;
   for (int64_t base = (int64_t)gwave * kWave; base < p.V; base += nthr) {
     const int32_t u = (int32_t)(base + lane);
-    unsigned long long acc = 0, need = 0, vx = 0, vo = 0;
+    unsigned long long accS = 0, accT = 0, needS = 0, needT = 0, vS = 0, vT = 0;
     OffT b = 0, e = 0;
     if (u < p.V) {
-      vx = p.rec[4 * (int64_t)u + 2 * x];
-      need = act & ~vx;
-      if (need) {
+      const ulonglong2* r = reinterpret_cast<const ulonglong2*>(p.rec + 4 * (int64_t)u);
+      vS = r[0].x;
+      vT = r[1].x;
+      if (DS) needS = act & ~vS;
+      if (DT) needT = act & ~vT;
+      if (needS | needT) {
         b = p.off[u];
         e = p.off[u + 1];
       }
     }
     if (b < e) {
-      a_cnt += 1;
+      red[kRActV] += 1;
       for (OffT j = b; j < e; j += 4) {
-        const int32_t c0 = p.col[j];
-        const int32_t c1 = j + 1 < e ? p.col[j + 1] : -1;
-        const int32_t c2 = j + 2 < e ? p.col[j + 2] : -1;
-        const int32_t c3 = j + 3 < e ? p.col[j + 3] : -1;
-        acc |= F[c0];
-        if (c1 >= 0) acc |= F[c1];
-        if (c2 >= 0) acc |= F[c2];
-        if (c3 >= 0) acc |= F[c3];
-        e_cnt += (unsigned long long)((j + 4 <= e) ? 4 : (e - j));
-        if ((acc & need) == need) break;
+        int32_t c[4];
+        c[0] = p.col[j];
+        c[1] = j + 1 < e ? p.col[j + 1] : -1;
+        c[2] = j + 2 < e ? p.col[j + 2] : -1;
+        c[3] = j + 3 < e ? p.col[j + 3] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (c[k] < 0) continue;
+          if (DS) accS |= FS[c[k]];
+          if (DT) accT |= FT[c[k]];
+        }
+        red[kRActE] += (unsigned long long)((j + 4 <= e) ? 4 : (e - j));
+        if ((accS & needS) == needS && (accT & needT) == needT) break;
       }
     }
-    const unsigned long long n = acc & need;
+    const unsigned long long nS = accS & needS, nT = accT & needT;
     int64_t deg = 0;
-    if (n) {
-      vo = p.rec[4 * (int64_t)u + 2 * (1 - x)];
-      p.rec[4 * (int64_t)u + 2 * x] = vx | n;
-      p.F[x][nxt][u] = n;
-      red[0] |= n & vo;
-      red[2 + x] |= n;
+    if (nS | nT) {
+      ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
+      if (nS) r[0].x = vS | nS;
+      if (nT) r[1].x = vT | nT;
+      if (nS) p.F[0][nxt[0]][u] = nS;
+      if (nT) p.F[1][nxt[1]][u] = nT;
+      red[kROdd] |= (nS & vT) | (nT & vS);
+      red[kREven] |= nS & nT;
+      red[kRAnyS] |= nS;
+      red[kRAnyT] |= nT;
       deg = (int64_t)(e - b);
-      red[4 + x] += (unsigned long long)deg;
+      if (nS) red[kRMfS] += (unsigned long long)deg;
+      if (nT) red[kRMfT] += (unsigned long long)deg;
     }
-    if (x == 0)
-      push_frontier(p, s0, 0, nxt, n != 0, u, deg);
-    else
-      push_frontier(p, s1, 1, nxt, n != 0, u, deg);
-    stage_push(s2, n != 0 && vx == 0 && vo == 0, u, &p.ctl->dirty_n, p.dirty, (uint32_t)p.V + 256u, &p.ctl->err);
+    if (DS) push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
+    if (DT) push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
+    // vertices base .. base+63 are this wave's alone in this launch
+    mark_dirty(p, (nS | nT) != 0 && (vS | vT) == 0, base >> 5, (base >> 5) + 1);
   }
 }
 
-// Features of the pending step for the learned scheduler.
-__device__ __forceinline__ void batch_features(BatchCtl* ctl, int64_t V, int64_t E) {
-  uint32_t n[2];
-  unsigned long long mf[2], hn[2];
+// Filter: the current frontier lists of side x still hold vertices of
+// queries resolved since they were built. Keep each vertex's mask restricted
+// to the live queries in the other parity, clear the old mask, and rebuild
+// the lists (and the hub edge bases) from the survivors only.
+template <typename OffT, int x>
+__device__ __forceinline__ void filter_side(const BatchParams<OffT>& p, Appends& ap, unsigned long long act,
+                                            int gwave, int nwaves, unsigned long long* red) {
+  const BatchCtl* ctl = p.ctl;
+  const int c = ctl->cur[x], nx = c ^ 1;
+  const int lane = lane_id();
+  unsigned long long* Fc = p.F[x][c];
+  unsigned long long* Fn = p.F[x][nx];
+  uint32_t n = ctl->fl_n[x][c];
+  if (n > (uint32_t)p.V) n = (uint32_t)p.V;
+  uint32_t nh = (uint32_t)(ctl->hub[x][c] >> kHubEdgeBits);
+  if (nh > p.hcap) nh = p.hcap;
+  const uint32_t tot = n + nh;
+  for (uint32_t base = (uint32_t)gwave * kWave; base < tot; base += (uint32_t)nwaves * kWave) {
+    const uint32_t i = base + (uint32_t)lane;
+    int32_t v = -1;
+    if (i < tot) v = i < n ? p.fl[x][c][i] : p.hl[x][c][i - n];
+    unsigned long long keep = 0;
+    int64_t deg = 0;
+    if ((uint32_t)v < (uint32_t)p.V) {
+      const unsigned long long fm = Fc[v];
+      Fc[v] = 0ull;
+      keep = fm & act;
+      if (keep) {
+        Fn[v] = keep;
+        deg = bdeg(p, v);
+        red[kRMfS + x] += (unsigned long long)deg;
+      }
+    }
+    push_frontier(p, ap, x, nx, keep != 0, v, deg);
+  }
+}
+
+// Features of the pending step for the learned scheduler (12 values, each
+// scaled to about [0, 1]): per side the frontier size and degree sum over
+// the live queries, their mean degree and the side's level; the graph size;
+// the live-query fraction and the hub share of the frontiers.
+__device__ __forceinline__ void batch_features(BatchCtl* ctl, const uint64_t* mf, int64_t V, int64_t E) {
+  auto lg = [](double x) { return (float)log2(x + 1.0); };
+  float* f = ctl->feat;
+  const int built[2] = {(int)__popcll(ctl->lact[0]), (int)__popcll(ctl->lact[1])};
+  const int live[2] = {(int)__popcll(ctl->lact[0] & ctl->act), (int)__popcll(ctl->lact[1] & ctl->act)};
+  double n[2];
+  uint64_t hn = 0;
   for (int x = 0; x < 2; ++x) {
     const int c = ctl->cur[x];
-    hn[x] = ctl->hub[x][c] >> kHubEdgeBits;
-    n[x] = ctl->fl_n[x][c] + (uint32_t)hn[x];
-    mf[x] = ctl->fl_mf[x][c];
+    hn += ctl->hub[x][c] >> kHubEdgeBits;
+    n[x] = (double)ctl->fl_n[x][c] + (double)(ctl->hub[x][c] >> kHubEdgeBits);
+    if (built[x] > 0) n[x] *= (double)live[x] / (double)built[x];
   }
-  const uint64_t mfs[2] = {mf[0], mf[1]};
-  const uint64_t hns[2] = {hn[0], hn[1]};
-  step_features(ctl->feat, n, mfs, hns, ctl->lvl[0], ctl->lvl[1], V, E, 0);
+  f[0] = lg(n[0]) / 32.f;
+  f[1] = lg((double)mf[0]) / 40.f;
+  f[2] = lg(n[1]) / 32.f;
+  f[3] = lg((double)mf[1]) / 40.f;
+  f[4] = lg((double)E) / 40.f;
+  f[5] = lg((double)V) / 32.f;
+  f[6] = (float)(ctl->lvl[0] < 64 ? ctl->lvl[0] : 64) / 64.f;
+  f[7] = (float)(ctl->lvl[1] < 64 ? ctl->lvl[1] : 64) / 64.f;
+  f[8] = lg(n[0] > 0 ? (double)mf[0] / n[0] : 0.0) / 20.f;
+  f[9] = lg(n[1] > 0 ? (double)mf[1] / n[1] : 0.0) / 20.f;
+  f[10] = (float)__popcll(ctl->act) / 64.f;
+  const double ntot = (double)ctl->fl_n[0][ctl->cur[0]] + (double)ctl->fl_n[1][ctl->cur[1]] + (double)hn;
+  f[11] = (float)((double)hn / (ntot + 1.0));
 }
 
-__device__ __forceinline__ int bdegree_width(uint32_t n, uint64_t mf) {
+__device__ __forceinline__ int bdegree_width(uint64_t n, uint64_t mf) {
   if (n == 0) return 4;
   const uint64_t avg = (mf + n - 1) / n;
   int vw = 4;
@@ -431,111 +595,134 @@ __device__ __forceinline__ int bdegree_width(uint32_t n, uint64_t mf) {
 }
 
 __device__ __forceinline__ void batch_apply_action(BatchDecision& d, int a) {
-  d.sides = (a / 3) == 0 ? kSideS : kSideT;
-  d.strat = (a % 3) == 2 ? kBottomUp : kTopDown;
-  d.vw = (a % 3) == 1 ? 64 : 8;
+  const int k = a % 3;
+  d.sides = k == 0 ? kSideS : k == 1 ? kSideT : kSideBoth;
+  d.strat = a < 3 ? kTopDown : kBottomUp;
   d.action = a;
 }
 
-// Next step of the current wave (one thread of the finalising block).
-__device__ __forceinline__ void batch_decide(BatchCtl* ctl, const float* mlp, int64_t V, int64_t E) {
+constexpr uint64_t kBothMaxEdges = 1u << 15;  // heuristic: expand both sides while both frontiers are this small
+
+// Degree sum of side x's current frontier restricted to the live queries,
+// estimated from the fraction of the queries the lists were built for that
+// are still live (exact right after a filter).
+__device__ __forceinline__ uint64_t live_mf(const BatchCtl* ctl, int x) {
+  const uint64_t mf = ctl->fl_mf[x][ctl->cur[x]];
+  const int built = __popcll(ctl->lact[x]);
+  const int live = __popcll(ctl->lact[x] & ctl->act);
+  if (built == 0 || live == built) return mf;
+  return (uint64_t)((double)mf * (double)live / (double)built);
+}
+
+// Heuristic (or plain) action for the pending step; also fills the feature
+// vector f for the learned policy. One lane of the finalising wave.
+__device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t E, float* f) {
+  uint64_t mf[2];
+  for (int x = 0; x < 2; ++x) mf[x] = live_mf(ctl, x);
+  batch_features(ctl, mf, V, E);
+  for (int k = 0; k < 12; ++k) f[k] = ctl->feat[k];
+  if (ctl->policy == 0) return 2;  // plain: both sides top-down
+  if (mf[0] + mf[1] <= kBothMaxEdges) return 2;
+  const int side = mf[0] <= mf[1] ? 0 : 1;
+  // A bit-parallel bottom-up vertex stops only once it has found every live
+  // query's bit: worth it only when the top-down frontier would touch a large
+  // part of the graph anyway.
+  return side + ((double)mf[side] * 4.0 > (double)E ? 3 : 0);
+}
+
+// Turn action a into the step decision and the next phase, filtering first
+// the lists the step will read when they carry resolved queries and are
+// large enough for that to pay.
+__device__ __forceinline__ void batch_apply(BatchCtl* ctl, int a, bool allow_filter) {
   BatchDecision d;
-  unsigned long long mf[2];
-  uint32_t nn[2];
+  batch_apply_action(d, a);
+  uint64_t n = 0, m = 0;
   for (int x = 0; x < 2; ++x) {
+    if (!(d.sides & (1 << x))) continue;
     const int c = ctl->cur[x];
-    mf[x] = ctl->fl_mf[x][c];
-    nn[x] = ctl->fl_n[x][c];
+    n += ctl->fl_n[x][c];
+    m += ctl->fl_mf[x][c] - (ctl->hub[x][c] & kHubEdgeMask);
   }
-  if (ctl->policy == 0) {
-    d.sides = kSideBoth;
-    d.strat = kTopDown;
-    d.action = -1;
-    const uint32_t n = nn[0] + nn[1];
-    const unsigned long long m = (mf[0] - (ctl->hub[0][ctl->cur[0]] & kHubEdgeMask)) +
-                                 (mf[1] - (ctl->hub[1][ctl->cur[1]] & kHubEdgeMask));
-    d.vw = bdegree_width(n, m);
-  } else {
-    const int side = mf[0] <= mf[1] ? 0 : 1;
-    // A bit-parallel bottom-up vertex can stop only once it has found every
-    // live query's bit, so it scans nearly its whole list: worth it only when
-    // the top-down frontier would touch most of the graph anyway.
-    const bool bu = (double)mf[side] * 2.0 > (double)E;
-    const int w = bdegree_width(nn[side], mf[side] - (ctl->hub[side][ctl->cur[side]] & kHubEdgeMask));
-    int a = side * 3 + (bu ? 2 : (w >= 16 ? 1 : 0));
-    batch_features(ctl, V, E);
-    if (ctl->policy == 2 && mlp) a = mlp_argmin(mlp, ctl->feat);
-    batch_apply_action(d, a);
-    if (d.strat == kTopDown) d.vw = w;
-    if (ctl->policy == 2 && (a % 3) != 2) d.vw = (a % 3) == 1 ? 64 : 8;
-  }
-  if (ctl->vw_fixed > 0 && d.strat == kTopDown) d.vw = ctl->vw_fixed;
+  d.vw = bdegree_width(n, m);
+  if (ctl->vw_fixed > 0) d.vw = ctl->vw_fixed;
   ctl->dec = d;
+  int filt = 0;
+  if (allow_filter) {
+    for (int x = 0; x < 2; ++x) {
+      if (!(d.sides & (1 << x))) continue;
+      const int c = ctl->cur[x];
+      const uint64_t size = (uint64_t)ctl->fl_n[x][c] + (ctl->hub[x][c] >> kHubEdgeBits);
+      if ((ctl->lact[x] & ~ctl->act) && (size >= kFilterMinVertices || ctl->fl_mf[x][c] >= kFilterMinEdges))
+        filt |= 1 << x;
+    }
+  }
+  if (filt) {
+    ctl->filt = filt;
+    ctl->phase = kBFilter;
+    return;
+  }
+  ctl->phase = d.strat == kBottomUp ? kBBottomUp : kBExpand;
+}
+
+// Return codes of batch_finalize: what the finalising wave does next.
+enum : int { kFinDone = 0, kFinDecide = 1, kFinDecideFilter = 2 };
+
+// After a step: clean up when the wave is done, else ask for a decision.
+__device__ __forceinline__ int batch_next(BatchCtl* ctl, bool allow_filter) {
+  if (!ctl->act) {
+    ctl->phase = kBCleanup;
+    return kFinDone;
+  }
+  return allow_filter ? kFinDecideFilter : kFinDecide;
 }
 
 template <typename OffT>
-__device__ __forceinline__ void batch_finalize(const BatchParams<OffT>& p, int phase, const BatchDecision& d) {
+__device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int phase, const BatchDecision& d,
+                                              const RedTot& r) {
   BatchCtl* ctl = p.ctl;
   const int wave0 = ctl->wave * 64;
+  ctl->active_vertices += (int64_t)r.v[kRActV];
+  ctl->active_edges += (int64_t)r.v[kRActE];
+  ctl->touched_total += (int64_t)r.v[kRTouched];
   switch (phase) {
     case kBInit: {
-      unsigned long long act = 0;
-      for (int q = 0; q < 64; ++q) {
-        const int qi = wave0 + q;
-        if (qi >= ctl->nq) break;
-        if (p.qs[qi] == p.qt[qi])
-          p.dist[qi] = 0;
-        else
-          act |= 1ull << q;
-      }
+      const unsigned long long act = ld_agent(&ctl->act);
       ctl->act = act;
       ctl->lvl[0] = ctl->lvl[1] = 0;
       for (int x = 0; x < 2; ++x) {
-        const int c = ctl->cur[x];
-        ctl->fl_mf[x][c] = ld_agent(&ctl->fl_mf[x][c]);
+        ctl->fl_n[x][0] = ld_agent(&ctl->fl_n[x][0]);
+        ctl->hub[x][0] = ld_agent(&ctl->hub[x][0]);
+        ctl->fl_mf[x][0] = ld_agent(&ctl->fl_mf[x][0]);
+        ctl->lact[x] = act;
       }
-      if (!act) {
-        ctl->phase = kBCleanup;
-        return;
-      }
-      batch_decide(ctl, p.mlp, p.V, p.E);
-      ctl->phase = ctl->dec.strat == kBottomUp ? kBBottomUp : kBExpand;
-      return;
+      return batch_next(ctl, false);
     }
     case kBExpand:
       ctl->phase = kBCommit;
-      return;
+      return kFinDone;
     case kBCommit:
     case kBBottomUp: {
       const unsigned long long act = ctl->act;
-      const unsigned long long odd = ld_agent(&ctl->odd) & act;
-      const unsigned long long even = ld_agent(&ctl->even) & act & ~odd;
-      const unsigned long long anys = ld_agent(&ctl->any_s), anyt = ld_agent(&ctl->any_t);
-      const bool es = d.sides & kSideS, et = d.sides & kSideT;
+      const unsigned long long odd = r.v[kROdd] & act;
+      const unsigned long long even = r.v[kREven] & act & ~odd;
       unsigned long long alive = act;
-      if (es) alive &= anys;
-      if (et) alive &= anyt;
+      if (d.sides & kSideS) alive &= r.v[kRAnyS];
+      if (d.sides & kSideT) alive &= r.v[kRAnyT];
       const unsigned long long dead = act & ~alive & ~odd & ~even;
       const int base = ctl->lvl[0] + ctl->lvl[1];
-      for (int q = 0; q < 64; ++q) {
+      for (unsigned long long m = odd | even | dead; m; m &= m - 1) {
+        const int q = __builtin_ctzll(m);
         const unsigned long long b = 1ull << q;
-        if (odd & b)
-          p.dist[wave0 + q] = base + 1;
-        else if (even & b)
-          p.dist[wave0 + q] = base + 2;
-        else if (dead & b)
-          p.dist[wave0 + q] = -1;
+        p.dist[wave0 + q] = (odd & b) ? base + 1 : (even & b) ? base + 2 : -1;
       }
       ctl->act = act & ~(odd | even | dead);
-      ctl->odd = ctl->even = ctl->any_s = ctl->any_t = 0;
-      ctl->touched_total += ctl->touched_n;
-      ctl->touched_n = 0;
       ctl->steps++;
       ctl->levels_total++;
       for (int x = 0; x < 2; ++x) {
         if (!(d.sides & (1 << x))) continue;
         const int c = ctl->cur[x];
         ctl->lvl[x]++;
+        ctl->lact[x] = act;
         // The consumed parity becomes the next "next": its counters restart.
         // After a bottom-up step its lists are still needed to clear the
         // masks, so kBClearF restarts them.
@@ -545,32 +732,42 @@ __device__ __forceinline__ void batch_finalize(const BatchParams<OffT>& p, int p
         }
         ctl->fl_mf[x][c] = 0;
         ctl->cur[x] = c ^ 1;
-        ctl->fl_mf[x][c ^ 1] = ld_agent(&ctl->fl_mf[x][c ^ 1]);
+        ctl->fl_n[x][c ^ 1] = ld_agent(&ctl->fl_n[x][c ^ 1]);
+        ctl->hub[x][c ^ 1] = ld_agent(&ctl->hub[x][c ^ 1]);
+        ctl->fl_mf[x][c ^ 1] = r.v[kRMfS + x];
       }
       if (phase == kBBottomUp) {
-        ctl->bu_side = (d.sides & kSideS) ? 0 : 1;
+        ctl->bu_sides = d.sides;
         ctl->phase = kBClearF;  // the consumed masks are cleared first
-        return;
+        return kFinDone;
       }
-      if (!ctl->act) {
-        ctl->phase = kBCleanup;
-        return;
-      }
-      batch_decide(ctl, p.mlp, p.V, p.E);
-      ctl->phase = ctl->dec.strat == kBottomUp ? kBBottomUp : kBExpand;
-      return;
+      return batch_next(ctl, true);
     }
     case kBClearF: {
-      const int x = ctl->bu_side;
-      ctl->fl_n[x][ctl->cur[x] ^ 1] = 0;
-      ctl->hub[x][ctl->cur[x] ^ 1] = 0;
-      if (!ctl->act) {
-        ctl->phase = kBCleanup;
-        return;
+      for (int x = 0; x < 2; ++x) {
+        if (!(ctl->bu_sides & (1 << x))) continue;
+        ctl->fl_n[x][ctl->cur[x] ^ 1] = 0;
+        ctl->hub[x][ctl->cur[x] ^ 1] = 0;
       }
-      batch_decide(ctl, p.mlp, p.V, p.E);
-      ctl->phase = ctl->dec.strat == kBottomUp ? kBBottomUp : kBExpand;
-      return;
+      return batch_next(ctl, true);
+    }
+    case kBFilter: {
+      const unsigned long long act = ctl->act;
+      for (int x = 0; x < 2; ++x) {
+        if (!(ctl->filt & (1 << x))) continue;
+        const int c = ctl->cur[x];
+        ctl->fl_n[x][c] = 0;
+        ctl->hub[x][c] = 0;
+        ctl->fl_mf[x][c] = 0;
+        ctl->cur[x] = c ^ 1;
+        ctl->fl_n[x][c ^ 1] = ld_agent(&ctl->fl_n[x][c ^ 1]);
+        ctl->hub[x][c ^ 1] = ld_agent(&ctl->hub[x][c ^ 1]);
+        ctl->fl_mf[x][c ^ 1] = r.v[kRMfS + x];
+        ctl->lact[x] = act;
+      }
+      ctl->filt = 0;
+      ctl->filters++;
+      return batch_next(ctl, false);
     }
     case kBCleanup: {
       for (int x = 0; x < 2; ++x) {
@@ -581,19 +778,81 @@ __device__ __forceinline__ void batch_finalize(const BatchParams<OffT>& p, int p
         }
         ctl->cur[x] = 0;
       }
-      ctl->dirty_n = 0;
-      ctl->touched_n = 0;
+      ctl->act = 0;
       ctl->wave++;
       ctl->phase = ctl->wave < ctl->nwaves ? kBInit : kBIdle;
-      return;
+      return kFinDone;
     }
     default:
       ctl->phase = kBIdle;
   }
+  return kFinDone;
+}
+
+// Seed a wave (block 0, waves 0 and 1 = sides s and t, one lane per query).
+// Records, masks and the dirty bitmap of the roots are zero here, so plain
+// stores suffice; repeated roots within a side are merged in LDS first. The
+// counters are written with agent-scope stores for the finalising block.
+template <typename OffT>
+__device__ __forceinline__ void init_wave(const BatchParams<OffT>& p, int32_t* s_root) {
+  BatchCtl* ctl = p.ctl;
+  const int q = lane_id(), x = (int)(threadIdx.x >> 6);
+  const int qi = ctl->wave * 64 + q;
+  bool live = false;
+  int32_t s = 0, t = 0;
+  if (qi < ctl->nq) {
+    s = p.qs[qi];
+    t = p.qt[qi];
+    live = s != t && (uint32_t)s < (uint32_t)p.V && (uint32_t)t < (uint32_t)p.V;
+    if (x == 0 && s == t) p.dist[qi] = 0;
+  }
+  const uint64_t lm = ballot(live);
+  const int32_t u = live ? (x ? t : s) : -1;
+  s_root[x * 64 + q] = u;
+  __builtin_amdgcn_wave_barrier();
+  unsigned long long mask = 0;
+  bool owner = live;
+  if (live) {
+    for (int j = 0; j < 64; ++j) {
+      if (s_root[x * 64 + j] == u) {
+        mask |= 1ull << j;
+        if (j < q) owner = false;
+      }
+    }
+  }
+  int64_t deg = 0;
+  if (owner) {
+    p.rec[4 * (int64_t)u + 2 * x] = mask;
+    p.F[x][0][u] = mask;
+    atomicOr(&p.db[u >> 5], 1u << (u & 31));
+    deg = bdeg(p, u);
+  }
+  const bool hub = owner && deg >= p.hub_threshold;
+  const uint64_t nm = ballot(owner && !hub), hm = ballot(hub);
+  if (owner && !hub) p.fl[x][0][lane_rank(nm)] = u;
+  const unsigned long long hd = hub ? (unsigned long long)deg : 0ull;
+  const unsigned long long incl = wave_incl_scan_u64(hd);
+  if (hub) {
+    const uint32_t slot = lane_rank(hm);
+    if (slot < p.hcap) {
+      p.hl[x][0][slot] = u;
+      p.hb[x][0][slot] = (int64_t)(incl - hd);
+    }
+  }
+  const unsigned long long mf = wave_sum_u64(owner ? (unsigned long long)deg : 0ull);
+  const unsigned long long hsum = wave_sum_u64(hd);
+  if (q == 0) {
+    st_agent(&ctl->fl_n[x][0], (uint32_t)__popcll(nm));
+    const uint32_t nh = (uint32_t)__popcll(hm);
+    if (nh > p.hcap) atomicOr(&ctl->err, 1u);
+    st_agent(&ctl->hub[x][0], ((unsigned long long)nh << kHubEdgeBits) | hsum);
+    st_agent(&ctl->fl_mf[x][0], mf);
+    if (x == 0) st_agent(&ctl->act, (unsigned long long)lm);
+  }
 }
 
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void batch_kernel(BatchParams<OffT> p) {
+__global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
   BatchCtl* ctl = p.ctl;
   const int phase = ctl->phase;
   if (phase == kBIdle) return;
@@ -602,66 +861,31 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchParams<OffT> p) {
   const int nwaves = (int)(nb * kWavesPerBlock);
   const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t nthr = (int64_t)nb * kBlock;
-  __shared__ int32_t s_stage[kBLists][kWavesPerBlock * kBStage];
+  __shared__ int32_t s_stage[2][kWavesPerBlock * kBStage];
+  __shared__ int32_t s_hub[2][2][kWavesPerBlock * kHubStage];
   __shared__ BShared sh;
   const BatchDecision d = ctl->dec;
   const unsigned long long act = ctl->act;
   const int w = (int)(threadIdx.x >> 6);
-  Stage s0{&s_stage[0][w * kBStage], 0u}, s1{&s_stage[1][w * kBStage], 0u}, s2{&s_stage[2][w * kBStage], 0u};
+  const int nxt[2] = {ctl->cur[0] ^ 1, ctl->cur[1] ^ 1};
+  Appends ap{{{&s_stage[0][w * kBStage], 0u}, {&s_stage[1][w * kBStage], 0u}},
+             {{&s_hub[0][0][w * kHubStage], &s_hub[0][1][w * kHubStage], 0u},
+              {&s_hub[1][0][w * kHubStage], &s_hub[1][1][w * kHubStage], 0u}}};
+  unsigned long long red[kRCount];
+#pragma unroll
+  for (int k = 0; k < kRCount; ++k) red[k] = 0;
 
   switch (phase) {
     case kBInit: {
-      // Seed the wave: one thread per (query, side), block 0 only.
-      if (blockIdx.x == 0 && threadIdx.x < 128) {
-        const int q = (int)(threadIdx.x & 63), x = (int)(threadIdx.x >> 6);
-        const int qi = ctl->wave * 64 + q;
-        if (qi < ctl->nq) {
-          const int32_t s = p.qs[qi], t = p.qt[qi];
-          if (s != t && (uint32_t)s < (uint32_t)p.V && (uint32_t)t < (uint32_t)p.V) {
-            const int32_t u = x ? t : s;
-            const unsigned long long b = 1ull << q;
-            const unsigned long long oldv = atomicOr(&p.rec[4 * (int64_t)u + 2 * x], b);
-            if (oldv == 0) {
-              const uint32_t k = atomicAdd(&ctl->dirty_n, 1u);
-              if (k < (uint32_t)p.V + 256u) p.dirty[k] = u;
-            }
-            const unsigned long long oldf = atomicOr(&p.F[x][0][u], b);
-            if (oldf == 0) {
-              const int64_t deg = bdeg(p, u);
-              if (deg >= p.hub_threshold) {
-                const unsigned long long o =
-                    atomicAdd(&ctl->hub[x][0], (1ull << kHubEdgeBits) | (unsigned long long)deg);
-                const uint32_t hs = (uint32_t)(o >> kHubEdgeBits);
-                if (hs < p.hcap) {
-                  p.hl[x][0][hs] = u;
-                  p.hb[x][0][hs] = (int64_t)(o & kHubEdgeMask);
-                }
-              } else {
-                const uint32_t k = atomicAdd(&ctl->fl_n[x][0], 1u);
-                if (k < (uint32_t)p.V) p.fl[x][0][k] = u;
-              }
-              atomicAdd(&ctl->fl_mf[x][0], (unsigned long long)deg);
-            }
-          }
-        }
-      }
+      if (blockIdx.x == 0 && threadIdx.x < 128) init_wave(p, &s_stage[0][0]);
       break;
     }
     case kBExpand: {
-      unsigned long long a_cnt = 0, e_cnt = 0;
       const int chunk = ctl->tile > 0 ? ctl->tile : 256;
 #pragma unroll
       for (int x = 0; x < 2; ++x)
-        if (d.sides & (1 << x)) expand_side(p, s0, x, d.vw, chunk, act, gwave, nwaves, a_cnt, e_cnt);
-      uint32_t* counters[1] = {&ctl->touched_n};
-      int32_t* lists[1] = {p.touched};
-      const uint32_t caps[1] = {(uint32_t)p.V};
-      stages_block_flush(s0, s1, s2, 1, counters, lists, caps, &ctl->err, sh);
-      unsigned long long vals[2] = {a_cnt, e_cnt};
-      const int op[2] = {0, 0};
-      unsigned long long* dst[2] = {(unsigned long long*)&ctl->active_vertices,
-                                    (unsigned long long*)&ctl->active_edges};
-      block_reduce_apply(vals, op, 2, dst, sh);
+        if (d.sides & (1 << x)) expand_side(p, x, d.vw, chunk, act, gwave, nwaves, red[kRActV], red[kRActE]);
+      block_reduce_slots(red, p.red, sh);
       break;
     }
     case kBCommit: {
@@ -676,75 +900,33 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchParams<OffT> p) {
           if ((uint32_t)v < (uint32_t)p.V) p.F[x][c][v] = 0ull;
         }
       }
-      const int nxt[2] = {ctl->cur[0] ^ 1, ctl->cur[1] ^ 1};
-      unsigned long long red[6] = {0, 0, 0, 0, 0, 0};
-      uint32_t n = ctl->touched_n;
-      if (n > (uint32_t)p.V) n = (uint32_t)p.V;
-      for (int64_t base = (int64_t)gwave * kWave; base < n; base += (int64_t)nwaves * kWave) {
-        const int64_t i = base + lane_id();
-        const bool valid = i < n;
-        int32_t u = valid ? p.touched[i] : 0;
-        const bool ok = valid && (uint32_t)u < (uint32_t)p.V;
-        commit_vertex(p, s0, s1, s2, ok, u, act, nxt, red);
-      }
-      uint32_t* counters[3] = {&ctl->fl_n[0][nxt[0]], &ctl->fl_n[1][nxt[1]], &ctl->dirty_n};
-      int32_t* lists[3] = {p.fl[0][nxt[0]], p.fl[1][nxt[1]], p.dirty};
-      const uint32_t caps[3] = {(uint32_t)p.V, (uint32_t)p.V, (uint32_t)p.V + 256u};
-      stages_block_flush(s0, s1, s2, 3, counters, lists, caps, &ctl->err, sh);
-      const int op[6] = {1, 1, 1, 1, 0, 0};
-      unsigned long long* dst[6] = {&ctl->odd, &ctl->even, &ctl->any_s, &ctl->any_t, &ctl->fl_mf[0][nxt[0]],
-                                    &ctl->fl_mf[1][nxt[1]]};
-      block_reduce_apply(red, op, 6, dst, sh);
+      commit_pass(p, ap, act, nxt, gwave, nwaves, red);
+      appends_block_flush(p, ap, nxt, sh);
+      block_reduce_slots(red, p.red, sh);
       break;
     }
     case kBBottomUp: {
-      const int x = (d.sides & kSideS) ? 0 : 1;
-      unsigned long long red[6] = {0, 0, 0, 0, 0, 0};
-      unsigned long long a_cnt = 0, e_cnt = 0;
-      bottom_up(p, s0, s1, s2, x, act, gwave, nwaves, red, a_cnt, e_cnt);
-      const int nx0 = ctl->cur[0] ^ 1, nx1 = ctl->cur[1] ^ 1;
-      uint32_t* counters[3] = {&ctl->fl_n[0][nx0], &ctl->fl_n[1][nx1], &ctl->dirty_n};
-      int32_t* lists[3] = {p.fl[0][nx0], p.fl[1][nx1], p.dirty};
-      const uint32_t caps[3] = {(uint32_t)p.V, (uint32_t)p.V, (uint32_t)p.V + 256u};
-      stages_block_flush(s0, s1, s2, 3, counters, lists, caps, &ctl->err, sh);
-      const int op[8] = {1, 1, 1, 1, 0, 0, 0, 0};
-      unsigned long long vals[8] = {red[0], red[1], red[2], red[3], red[4], red[5], a_cnt, e_cnt};
-      unsigned long long* dst[8] = {&ctl->odd, &ctl->even, &ctl->any_s, &ctl->any_t,
-                                    &ctl->fl_mf[0][ctl->cur[0] ^ 1], &ctl->fl_mf[1][ctl->cur[1] ^ 1],
-                                    (unsigned long long*)&ctl->active_vertices,
-                                    (unsigned long long*)&ctl->active_edges};
-      block_reduce_apply(vals, op, 8, dst, sh);
+      if (d.sides == kSideS)
+        bottom_up<OffT, 1>(p, ap, act, gwave, nwaves, red);
+      else if (d.sides == kSideT)
+        bottom_up<OffT, 2>(p, ap, act, gwave, nwaves, red);
+      else
+        bottom_up<OffT, 3>(p, ap, act, gwave, nwaves, red);
+      appends_block_flush(p, ap, nxt, sh);
+      block_reduce_slots(red, p.red, sh);
+      break;
+    }
+    case kBFilter: {
+      if (ctl->filt & 1) filter_side<OffT, 0>(p, ap, act, gwave, nwaves, red);
+      if (ctl->filt & 2) filter_side<OffT, 1>(p, ap, act, gwave, nwaves, red);
+      appends_block_flush(p, ap, nxt, sh);
+      block_reduce_slots(red, p.red, sh);
       break;
     }
     case kBClearF: {
-      const int x = ctl->bu_side;
-      const int c = ctl->cur[x] ^ 1;  // the parity consumed by the bottom-up step
-      const uint32_t n = ctl->fl_n[x][c] < (uint32_t)p.V ? ctl->fl_n[x][c] : (uint32_t)p.V;
-      for (int64_t i = gtid; i < n; i += nthr) {
-        const int32_t v = p.fl[x][c][i];
-        if ((uint32_t)v < (uint32_t)p.V) p.F[x][c][v] = 0ull;
-      }
-      uint32_t nh = (uint32_t)(ctl->hub[x][c] >> kHubEdgeBits);
-      if (nh > p.hcap) nh = p.hcap;
-      for (int64_t i = gtid; i < nh; i += nthr) {
-        const int32_t v = p.hl[x][c][i];
-        if ((uint32_t)v < (uint32_t)p.V) p.F[x][c][v] = 0ull;
-      }
-      break;
-    }
-    case kBCleanup: {
-      uint32_t nd = ctl->dirty_n;
-      if (nd > (uint32_t)p.V + 256u) nd = (uint32_t)p.V + 256u;
-      for (int64_t i = gtid; i < nd; i += nthr) {
-        const int32_t v = p.dirty[i];
-        if ((uint32_t)v < (uint32_t)p.V) {
-          ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)v);
-          r[0] = make_ulonglong2(0ull, 0ull);
-          r[1] = make_ulonglong2(0ull, 0ull);
-        }
-      }
       for (int x = 0; x < 2; ++x) {
-        const int c = ctl->cur[x];
+        if (!(ctl->bu_sides & (1 << x))) continue;
+        const int c = ctl->cur[x] ^ 1;  // the parity consumed by the bottom-up step
         const uint32_t n = ctl->fl_n[x][c] < (uint32_t)p.V ? ctl->fl_n[x][c] : (uint32_t)p.V;
         for (int64_t i = gtid; i < n; i += nthr) {
           const int32_t v = p.fl[x][c][i];
@@ -755,6 +937,41 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchParams<OffT> p) {
         for (int64_t i = gtid; i < nh; i += nthr) {
           const int32_t v = p.hl[x][c][i];
           if ((uint32_t)v < (uint32_t)p.V) p.F[x][c][v] = 0ull;
+        }
+      }
+      break;
+    }
+    case kBCleanup: {
+      // Every vertex with a non-zero record or frontier mask is dirty: clear
+      // the records and all four mask arrays of the dirty vertices in
+      // address order.
+      const int lane = lane_id();
+      const int64_t words = ((int64_t)p.V + 31) / 32;
+      for (int64_t wb = (int64_t)gwave * kWave; wb < words; wb += (int64_t)nwaves * kWave) {
+        const int64_t wi = wb + lane;
+        const uint32_t word = wi < words ? p.db[wi] : 0u;
+        if (word) p.db[wi] = 0u;
+        uint64_t nz = ballot(word != 0);
+        while (nz) {
+          const int k0 = __builtin_ctzll(nz);
+          nz &= nz - 1;
+          int k1 = -1;
+          if (nz) {
+            k1 = __builtin_ctzll(nz);
+            nz &= nz - 1;
+          }
+          const int mine = lane < 32 ? k0 : k1;
+          const uint32_t wv = (uint32_t)__shfl((int)word, mine < 0 ? 0 : mine);
+          const int64_t u = (wb + (mine < 0 ? 0 : mine)) * 32 + (lane & 31);
+          if (mine >= 0 && ((wv >> (lane & 31)) & 1u) && u < p.V) {
+            ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * u);
+            r[0] = make_ulonglong2(0ull, 0ull);
+            r[1] = make_ulonglong2(0ull, 0ull);
+            p.F[0][0][u] = 0ull;
+            p.F[0][1][u] = 0ull;
+            p.F[1][0][u] = 0ull;
+            p.F[1][1][u] = 0ull;
+          }
         }
       }
       break;
@@ -770,9 +987,36 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchParams<OffT> p) {
     sh.last = t == nb - 1;
   }
   __syncthreads();
-  if (!sh.last || threadIdx.x != 0) return;
-  batch_finalize(p, phase, d);
-  __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!sh.last || threadIdx.x >= kWave) return;
+  // Wave 0 of the last block finalises: slot sums, the state update (lane
+  // 0), and, when a step must be chosen, the decision (the learned policy
+  // evaluates its network across the wave).
+  int need = kFinDone;
+  {
+    const RedTot tot = collect_slots(p.red);
+    if (threadIdx.x == 0) {
+      if (ctl->phases < kPhaseLog) {
+        ctl->plog[ctl->phases] = (int8_t)(phase * 16 + (phase == kBExpand || phase == kBBottomUp ? d.sides : 0));
+        ctl->ptime[ctl->phases] = (int64_t)wall_clock64();
+      }
+      ctl->phases++;
+      need = batch_finalize(p, phase, d, tot);
+    }
+  }
+  need = __shfl(need, 0);
+  if (need != kFinDone) {
+    float f[12];
+    int a = 0;
+    if (threadIdx.x == 0) a = batch_heuristic(ctl, p.V, p.E, f);
+    a = __shfl(a, 0);
+    if (ctl->policy == 2 && p.mlp) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) f[k] = __shfl(f[k], 0);
+      a = mlp_argmin_wave(p.mlp, f);
+    }
+    if (threadIdx.x == 0) batch_apply(ctl, a, need == kFinDecideFilter);
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename OffT>
@@ -801,12 +1045,12 @@ struct BatchWorkspace {
   size_t bytes = 0;
   unsigned long long* rec = nullptr;
   unsigned long long* F[2][2] = {};
-  uint32_t* tb = nullptr;
+  uint8_t* tm = nullptr;
+  unsigned long long* red = nullptr;
   int32_t* fl[2][2] = {};
   int32_t* hl[2][2] = {};
   int64_t* hb[2][2] = {};
-  int32_t* touched = nullptr;
-  int32_t* dirty = nullptr;
+  uint32_t* db = nullptr;
   BatchCtl* ctl = nullptr;
   int32_t* qs = nullptr;
   int32_t* qt = nullptr;
@@ -847,18 +1091,18 @@ static fc_status batch_alloc(fc_graph* g) {
   o += al(32 * V);
   const size_t o_F = o;
   o += 4 * al(8 * V);
-  const size_t o_tb = o;
-  o += al(4 * ((V + 31) / 32));
+  const size_t o_tm = o;
+  o += al((V + 31) / 32 + 64);
+  const size_t o_red = o;
+  o += al(sizeof(unsigned long long) * kRedSlots * kRedWords);
   const size_t o_fl = o;
   o += 4 * al(4 * V);
   const size_t o_hl = o;
   o += 4 * al(4 * hcap);
   const size_t o_hb = o;
   o += 4 * al(8 * hcap);
-  const size_t o_touch = o;
-  o += al(4 * V);
-  const size_t o_dirty = o;
-  o += al(4 * (V + 256));
+  const size_t o_db = o;
+  o += al(4 * ((V + 63) / 32));
   const size_t o_ctl = o;
   o += al(sizeof(BatchCtl));
   b->bytes = o;
@@ -881,9 +1125,9 @@ static fc_status batch_alloc(fc_graph* g) {
       b->hb[x][c] = (int64_t*)(b->mem + o_hb + (size_t)(x * 2 + c) * al(8 * hcap));
     }
   }
-  b->tb = (uint32_t*)(b->mem + o_tb);
-  b->touched = (int32_t*)(b->mem + o_touch);
-  b->dirty = (int32_t*)(b->mem + o_dirty);
+  b->tm = (uint8_t*)(b->mem + o_tm);
+  b->red = (unsigned long long*)(b->mem + o_red);
+  b->db = (uint32_t*)(b->mem + o_db);
   b->ctl = (BatchCtl*)(b->mem + o_ctl);
   int occ = g->off64 ? batch_occupancy<int64_t>() : batch_occupancy<int32_t>();
   if (occ < 1) {
@@ -922,7 +1166,7 @@ static fc_status batch_queries(fc_graph* g, int32_t n) {
 }
 
 template <typename OffT>
-static BatchParams<OffT> batch_params(fc_graph* g, int force_action) {
+static BatchParams<OffT> batch_params(fc_graph* g) {
   BatchWorkspace* b = g->batch;
   BatchParams<OffT> p;
   std::memset(&p, 0, sizeof(p));
@@ -941,28 +1185,20 @@ static BatchParams<OffT> batch_params(fc_graph* g, int force_action) {
     }
   }
   p.hcap = g->hcap;
-  p.tb = b->tb;
-  p.touched = b->touched;
-  p.dirty = b->dirty;
+  p.tm = b->tm;
+  p.red = b->red;
+  p.db = b->db;
   p.ctl = b->ctl;
   p.qs = b->qs;
   p.qt = b->qt;
   p.dist = b->dist;
   p.mlp = g->d_mlp;
-  p.force_action = force_action;
   return p;
 }
 
-static hipError_t batch_launch_raw(fc_graph* g, int force_action) {
-  if (g->off64)
-    return launch_batch_phase<int64_t>(batch_params<int64_t>(g, force_action), g->batch->blocks, g->stream);
-  return launch_batch_phase<int32_t>(batch_params<int32_t>(g, force_action), g->batch->blocks, g->stream);
-}
-
-fc_status batch_launch_once(fc_graph* g, int force_action) {
-  hipError_t e = batch_launch_raw(g, force_action);
-  if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
-  return FC_OK;
+static hipError_t batch_launch_raw(fc_graph* g) {
+  if (g->off64) return launch_batch_phase<int64_t>(batch_params<int64_t>(g), g->batch->blocks, g->stream);
+  return launch_batch_phase<int32_t>(batch_params<int32_t>(g), g->batch->blocks, g->stream);
 }
 
 static fc_status batch_graph(fc_graph* g) {
@@ -970,7 +1206,7 @@ static fc_status batch_graph(fc_graph* g) {
   if (b->gexec) return FC_OK;
   FC_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
-  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = batch_launch_raw(g, -1);
+  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = batch_launch_raw(g);
   hipGraph_t gr = nullptr;
   hipError_t ce = hipStreamEndCapture(g->stream, &gr);
   if (le != hipSuccess) return hip_fail(le, "batch kernel capture");
@@ -993,11 +1229,14 @@ fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile) {
   init.vw_fixed = vw;
   init.tile = tile;
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
+  // the reduction slots are cleared by every finalisation; clear them here
+  // too in case an earlier batch stopped on an error
+  FC_HIP(hipMemsetAsync(b->red, 0, sizeof(unsigned long long) * kRedSlots * kRedWords, g->stream));
   return FC_OK;
 }
 
 // Run the phase machine until idle. use_graph: replay captured launches.
-static fc_status batch_drive(fc_graph* g, int32_t n, bool use_graph, float* ms) {
+static fc_status batch_drive(fc_graph* g, int32_t n, bool use_graph, float* ms, int64_t* launches_out) {
   BatchWorkspace* b = g->batch;
   const int64_t nw = (n + 63) / 64;
   int64_t expected = (int64_t)std::ceil((double)nw * b->est_launches) + 2;
@@ -1007,7 +1246,7 @@ static fc_status batch_drive(fc_graph* g, int32_t n, bool use_graph, float* ms) 
     int64_t want = std::max<int64_t>(expected - launched, 1);
     if (!use_graph || want < kGraphLaunches) {
       for (int64_t i = 0; i < want; ++i) {
-        hipError_t e = batch_launch_raw(g, -1);
+        hipError_t e = batch_launch_raw(g);
         if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
       }
       launched += want;
@@ -1027,11 +1266,12 @@ static fc_status batch_drive(fc_graph* g, int32_t n, bool use_graph, float* ms) 
   }
   FC_HIP(hipEventElapsedTime(ms, g->ev0, g->ev1));
   b->est_launches = 0.5 * b->est_launches + 0.5 * ((double)launched / (double)nw);
+  if (launches_out) *launches_out = launched;
   return FC_OK;
 }
 
 static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy, int vw, int tile,
-                           bool use_graph, float* ms, BatchCtl* ctl_out) {
+                           bool use_graph, float* ms, BatchCtl* ctl_out, int64_t* launches = nullptr) {
   BatchWorkspace* b = g->batch;
   fc_status st = batch_queries(g, n);
   if (st != FC_OK) return st;
@@ -1044,12 +1284,23 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
   st = batch_begin(g, n, policy, vw, tile);
   if (st != FC_OK) return st;
   FC_HIP(hipStreamSynchronize(g->stream));
-  st = batch_drive(g, n, use_graph, ms);
+  st = batch_drive(g, n, use_graph, ms, launches);
   if (st != FC_OK) return st;
   FC_HIP(hipMemcpyAsync(b->h_dist, b->dist, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
   if (ctl_out) FC_HIP(hipMemcpyAsync(ctl_out, b->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, g->stream));
   FC_HIP(hipStreamSynchronize(g->stream));
   if (ctl_out && ctl_out->err) return fail(FC_ERR_CAPACITY, "batch kernel: list guard tripped (err=%u)", ctl_out->err);
+  if (ctl_out && std::getenv("FASTCON_BATCH_LOG")) {
+    // phase sequence: I=init E=expand(s/t/b sides) C=commit U=bottom-up X=clear F=filter Z=cleanup
+    const char* nm = "?IECUXZF";
+    const int64_t np = std::min<int64_t>(ctl_out->phases, kPhaseLog);
+    for (int64_t i = 0; i < np; ++i) {
+      const int ph = ctl_out->plog[i] >> 4, sd = ctl_out->plog[i] & 15;
+      const double us = i ? (double)(ctl_out->ptime[i] - ctl_out->ptime[i - 1]) / 100.0 : 0.0;  // 100 MHz clock
+      std::fprintf(stderr, "%c%s:%.0f", nm[ph & 7], sd == 1 ? "s" : sd == 2 ? "t" : sd == 3 ? "b" : "", us);
+      std::fputc(ph == kBCleanup ? '\n' : ' ', stderr);
+    }
+  }
   return FC_OK;
 }
 
@@ -1103,6 +1354,8 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
     return fc::fail(FC_ERR_INVALID, "fc_stcon_batch: learned policy requested but no scheduler parameters are set");
   if (o.vw != 0 && o.vw != 4 && o.vw != 8 && o.vw != 16 && o.vw != 32 && o.vw != 64)
     return fc::fail(FC_ERR_INVALID, "fc_stcon_batch: vw must be 0, 4, 8, 16, 32 or 64");
+  if (o.tile != 0 && (o.tile < 64 || o.tile > 65536 || (o.tile % 64) != 0))
+    return fc::fail(FC_ERR_INVALID, "fc_stcon_batch: tile must be 0 or a multiple of 64 in [64, 65536]");
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (n == 0) return FC_OK;
   fc_status st = fcd::batch_alloc(g);
@@ -1121,7 +1374,8 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
   if (tile == 0) tile = 256;
   float ms = 0;
   BatchCtl c;
-  st = fcd::batch_run(g, s, t, n, o.policy, vw, tile, o.use_graph != 0, &ms, &c);
+  int64_t launches = 0;
+  st = fcd::batch_run(g, s, t, n, o.policy, vw, tile, o.use_graph != 0, &ms, &c, &launches);
   if (st != FC_OK) return st;
   for (int32_t i = 0; i < n; ++i) {
     const int32_t d = b->h_dist[i];
@@ -1137,6 +1391,9 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
     stats->touched_vertices = c.touched_total;
     stats->vw = vw;
     stats->tile = tile;
+    stats->filters = (int32_t)c.filters;
+    stats->phases = (int32_t)c.phases;
+    stats->launches = launches;
   }
   return FC_OK;
 }

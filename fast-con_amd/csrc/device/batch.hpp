@@ -13,18 +13,31 @@ enum BatchPhase : int32_t {
   kBInit = 1,      // seed the wave's 64 queries
   kBExpand = 2,    // top-down: frontier vertices OR their masks into neighbours
   kBCommit = 3,    // new masks -> visited, meets, next frontier lists
-  kBBottomUp = 4,  // bottom-up step of one side (expand + commit in one pass)
+  kBBottomUp = 4,  // bottom-up step (expand + commit in one pass)
   kBClearF = 5,    // clear the consumed frontier masks after a bottom-up step
-  kBCleanup = 6    // zero every mask the wave touched, write results
+  kBCleanup = 6,   // zero every mask the wave touched, write results
+  kBFilter = 7     // drop resolved queries from the current frontier lists
 };
 
 enum BatchSides : int32_t { kSideS = 1, kSideT = 2, kSideBoth = 3 };
 
 struct BatchDecision {
   int32_t sides;     // kSideS / kSideT / kSideBoth
-  int32_t strat;     // kTopDown / kBottomUp (bottom-up: one side only)
+  int32_t strat;     // kTopDown / kBottomUp
   int32_t vw;        // top-down virtual-warp width
-  int32_t action;    // scheduler action (0..5), -1 for the plain both-sides step
+  int32_t action;    // scheduler action: sides {s, t, both}[a % 3], a < 3 top-down, else bottom-up
+};
+
+constexpr int kPhaseLog = 1024;
+
+// Per-launch reductions go to kRedSlots copies (block b adds into slot
+// b % kRedSlots) so that the blocks of a launch do not all hit one address;
+// the finalising wave sums the slots and clears them.
+constexpr int kRedSlots = 16;
+constexpr int kRedWords = 16;  // words per slot (one 128-byte line)
+enum BatchRed : int32_t {
+  kROdd = 0, kREven, kRAnyS, kRAnyT,  // OR
+  kRMfS, kRMfT, kRActV, kRActE, kRTouched, kRCount  // sum
 };
 
 struct BatchCtl {
@@ -38,15 +51,14 @@ struct BatchCtl {
   int32_t lvl[2];          // levels done by each side in this wave
   int32_t steps;
   int32_t cur[2];          // parity of the current frontier list/mask per side
-  int32_t bu_side;         // side of the last bottom-up step (for kBClearF)
+  int32_t bu_sides;        // sides of the last bottom-up step (for kBClearF)
+  int32_t filt;            // sides (bit x) compacted by kBFilter
+  unsigned long long lact[2];  // live queries when side x's current lists were built
   BatchDecision dec;
   unsigned long long act;  // unresolved queries of the wave
   uint32_t fl_n[2][2];     // normal frontier list sizes [side][parity]
   unsigned long long hub[2][2];   // hub lists: (entries << 40) | edges
   unsigned long long fl_mf[2][2]; // degree sums of the frontier (normal + hub)
-  uint32_t touched_n;
-  uint32_t dirty_n;
-  unsigned long long odd, even, any_s, any_t;   // block-reduced results of a commit
   uint32_t ticket;
   uint32_t err;
   // statistics (whole batch)
@@ -54,7 +66,11 @@ struct BatchCtl {
   int64_t active_vertices;  // sum over steps of frontier vertices processed
   int64_t active_edges;     // sum over steps of their degrees
   int64_t touched_total;
+  int64_t filters;          // filter launches
+  int64_t phases;           // launches that did work
   float feat[12];
+  int8_t plog[kPhaseLog];   // phase of each working launch (diagnostics)
+  int64_t ptime[kPhaseLog];  // wall clock at its finalisation
 };
 
 template <typename OffT>
@@ -69,20 +85,19 @@ struct BatchParams {
   // top-down step: one 16-byte load gives a probe both words it needs.
   unsigned long long* rec;     // [4*V]
   unsigned long long* F[2][2]; // frontier masks [side][parity], V entries each
-  uint32_t* tb;                // touched bitmap, ceil(V/32) words
+  uint8_t* tm;                 // touched marks: one byte per 32 vertices whose N words may be non-zero
+  unsigned long long* red;     // [kRedSlots][kRedWords] launch reductions
+  uint32_t* db;                // dirty bitmap: visited words of u non-zero this wave
   int32_t* fl[2][2];           // frontier lists [side][parity], V entries each
   int32_t* hl[2][2];           // hub frontier lists (degree >= hub_threshold)
   int64_t* hb[2][2];           // their edge bases (ascending with slot)
   uint32_t hcap;
-  int32_t* touched;            // vertices whose N word became non-zero this step
-  int32_t* dirty;              // vertices whose visited words became non-zero this wave
   BatchCtl* ctl;
   const int32_t* qs;           // queries (device)
   const int32_t* qt;
   int32_t* dist;               // results (device), nq entries
   const float* mlp;
-  int32_t force_action;
-  int32_t pad;
+  int32_t pad[2];
 };
 
 template <typename OffT>

@@ -217,6 +217,9 @@ typedef struct fc_batch_stats {
   int64_t touched_vertices;    /* sum over waves of vertices whose masks were written */
   int32_t vw;                  /* width used */
   int32_t tile;                /* tile schedule used */
+  int32_t filters;             /* frontier-list filter passes (resolved queries dropped) */
+  int32_t phases;              /* kernel launches that did work */
+  int64_t launches;            /* kernel launches issued (phases + no-op tail launches) */
 } fc_batch_stats;
 
 /* Batched s–t: queries are run as bit-parallel waves of 64 (one uint64 mask

@@ -104,3 +104,29 @@ def test_batch_reuse_and_errors(fc, orc):
         with pytest.raises(fc.FastconError):
             g.stcon_batch([0], [1], vw=7)
         assert len(g.stcon_batch([], []).dist) == 0
+
+
+def _forced_params(fc, action):
+    """Scheduler parameters whose network always picks `action` (all weights
+    zero, lowest output bias on that action)."""
+    w = np.zeros(fc.SCHED_PARAMS, np.float32)
+    w[-fc.SCHED_ACTIONS + action] = -1.0
+    return w
+
+
+@pytest.mark.parametrize("action", range(6))
+def test_batch_every_action_is_exact(fc, orc, action):
+    """Each scheduler action (side s / t / both, top-down / bottom-up) forced
+    at every step still gives the oracle's verdicts and distances."""
+    cases = [("rmat13", fc.gen_rmat(13, 8, seed=31)), ("er-sparse", fc.gen_er(3000, 1.3, 8)),
+             ("grid", fc.gen_grid(60, 50, 0.04, 2))]
+    for name, hg in cases:
+        s, t = _pairs(hg, 150, action + 3)
+        s[:2] = t[:2]
+        exp = _oracle_dist(orc, hg, s, t)
+        with fc.Graph(hg) as g:
+            g.sched_set_params(_forced_params(fc, action))
+            say(f"batch forced action={action} {name}")
+            res = g.stcon_batch(s, t, policy=fc.POLICY_LEARNED, vw=0, tile=256)
+            bad = np.flatnonzero(res.dist != exp)
+            assert bad.size == 0, f"{name}: {bad[:5]} got {res.dist[bad[:5]]} want {exp[bad[:5]]}"
