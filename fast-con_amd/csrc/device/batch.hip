@@ -40,8 +40,10 @@ namespace fcd {
 using fc::fail;
 
 constexpr uint32_t kBStage = 512;  // staged entries per wave and list
-constexpr uint64_t kFilterMinVertices = 4096;   // list size worth a filter launch
-constexpr uint64_t kFilterMinEdges = 1u << 16;  // or frontier degree sum
+// A filter launch pays when the expected number of dropped list entries or
+// frontier edges reaches these.
+constexpr uint64_t kFilterMinVertices = 1u << 15;
+constexpr uint64_t kFilterMinEdges = 1u << 20;
 
 // ---------------------------------------------------------------------------
 // LDS-staged appends (one wave stage per list; block publishes with one atomic)
@@ -79,52 +81,13 @@ __device__ __forceinline__ void stage_push(Stage& s, bool want, int32_t v, uint3
 }
 
 struct BShared {
-  uint32_t cnt[2][kWavesPerBlock];
+  uint32_t cnt[4][kWavesPerBlock];              // staged entries: lists s, t, hubs s, t
+  unsigned long long hedge[2][kWavesPerBlock];  // staged hub edges per side
   uint32_t base[2];
+  unsigned long long hbase[2];
   unsigned long long red[kRCount][kWavesPerBlock];
-  HubShared hub[2];
   int last;
 };
-
-// Block-collective publication of the two staged frontier lists (all threads
-// call, uniform control flow).
-__device__ __forceinline__ void stage_block_part(Stage& st, int l, int32_t* list, uint32_t cap, uint32_t* err,
-                                                 BShared& sh) {
-  const int w = (int)(threadIdx.x >> 6);
-  const uint32_t base = sh.base[l] + sh.cnt[l][w];
-  for (uint32_t i = (uint32_t)lane_id(); i < st.cnt; i += kWave) {
-    const uint32_t slot = base + i;
-    if (slot < cap)
-      list[slot] = st.buf[i];
-    else
-      atomicOr(err, 1u);
-  }
-  st.cnt = 0;
-}
-
-__device__ __forceinline__ void stages_block_flush(Stage& s0, Stage& s1, uint32_t* c0, uint32_t* c1, int32_t* l0,
-                                                   int32_t* l1, uint32_t cap, uint32_t* err, BShared& sh) {
-  const int w = (int)(threadIdx.x >> 6);
-  if (lane_id() == 0) {
-    sh.cnt[0][w] = s0.cnt;
-    sh.cnt[1][w] = s1.cnt;
-  }
-  __syncthreads();
-  if (threadIdx.x < 2) {
-    const int l = (int)threadIdx.x;
-    uint32_t tot = 0;
-    for (int i = 0; i < kWavesPerBlock; ++i) {
-      const uint32_t c = sh.cnt[l][i];
-      sh.cnt[l][i] = tot;
-      tot += c;
-    }
-    sh.base[l] = tot ? atomicAdd(l ? c1 : c0, tot) : 0u;
-  }
-  __syncthreads();
-  stage_block_part(s0, 0, l0, cap, err, sh);
-  stage_block_part(s1, 1, l1, cap, err, sh);
-  __syncthreads();
-}
 
 template <typename OffT>
 __device__ __forceinline__ int64_t bdeg(const BatchParams<OffT>& p, int32_t u) {
@@ -163,19 +126,19 @@ struct RedTot {
   unsigned long long v[kRCount];
 };
 
-// Wave 0 of the finalising block: sum the slots and clear them.
+// Wave 0 of the finalising block: sum the slots and clear them. All loads
+// are issued before any is used.
 __device__ __forceinline__ RedTot collect_slots(unsigned long long* red) {
   const int lane = lane_id();
+  unsigned long long v[kRCount];
+  unsigned long long* row = red + (size_t)(lane < kRedSlots ? lane : 0) * kRedWords;
+#pragma unroll
+  for (int k = 0; k < kRCount; ++k) v[k] = lane < kRedSlots ? ld_agent(row + k) : 0ull;
   RedTot t;
 #pragma unroll
   for (int k = 0; k < kRCount; ++k) {
-    unsigned long long v = 0;
-    if (lane < kRedSlots) {
-      unsigned long long* a = red + (size_t)lane * kRedWords + k;
-      v = ld_agent(a);
-      if (v) *a = 0ull;
-    }
-    t.v[k] = k < kRMfS ? (unsigned long long)wave_or_u64(v) : (unsigned long long)wave_sum_u64(v);
+    if (v[k]) row[k] = 0ull;
+    t.v[k] = k < kRMfS ? (unsigned long long)wave_or_u64(v[k]) : (unsigned long long)wave_sum_u64(v[k]);
   }
   return t;
 }
@@ -197,14 +160,69 @@ __device__ __forceinline__ void push_frontier(const BatchParams<OffT>& p, Append
   hub_push(ap.h[x], hub, u, deg, &p.ctl->hub[x][nxt], p.hl[x][nxt], p.hb[x][nxt], p.hcap, &p.ctl->err);
 }
 
+// Block-collective publication of the staged appends (all threads call,
+// uniform control flow): one barrier to publish every wave's counts, the
+// four list reservations issued in parallel by threads 0..3, then each wave
+// writes its entries.
 template <typename OffT>
 __device__ __forceinline__ void appends_block_flush(const BatchParams<OffT>& p, Appends& ap, const int* nxt,
                                                     BShared& sh) {
   BatchCtl* ctl = p.ctl;
-  stages_block_flush(ap.s[0], ap.s[1], &ctl->fl_n[0][nxt[0]], &ctl->fl_n[1][nxt[1]], p.fl[0][nxt[0]],
-                     p.fl[1][nxt[1]], (uint32_t)p.V, &ctl->err, sh);
-  hub_block_flush(ap.h[0], &ctl->hub[0][nxt[0]], p.hl[0][nxt[0]], p.hb[0][nxt[0]], p.hcap, &ctl->err, sh.hub[0]);
-  hub_block_flush(ap.h[1], &ctl->hub[1][nxt[1]], p.hl[1][nxt[1]], p.hb[1][nxt[1]], p.hcap, &ctl->err, sh.hub[1]);
+  const int w = (int)(threadIdx.x >> 6);
+  const int lane = lane_id();
+  unsigned long long he[2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+    he[x] = wave_sum_u64((uint32_t)lane < ap.h[x].cnt ? (unsigned long long)ap.h[x].d[lane] : 0ull);
+  if (lane == 0) {
+    sh.cnt[0][w] = ap.s[0].cnt;
+    sh.cnt[1][w] = ap.s[1].cnt;
+    sh.cnt[2][w] = ap.h[0].cnt;
+    sh.cnt[3][w] = ap.h[1].cnt;
+    sh.hedge[0][w] = he[0];
+    sh.hedge[1][w] = he[1];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int l = (int)threadIdx.x;
+    uint32_t c = 0;
+    unsigned long long e = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      const uint32_t ci = sh.cnt[l][i];
+      sh.cnt[l][i] = c;
+      c += ci;
+      if (l >= 2) {
+        const unsigned long long ei = sh.hedge[l - 2][i];
+        sh.hedge[l - 2][i] = e;
+        e += ei;
+      }
+    }
+    if (l < 2)
+      sh.base[l] = c ? atomicAdd(&ctl->fl_n[l][nxt[l]], c) : 0u;
+    else
+      sh.hbase[l - 2] = c ? atomicAdd(&ctl->hub[l - 2][nxt[l - 2]], ((unsigned long long)c << kHubEdgeBits) | e) : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    Stage& st = ap.s[x];
+    const uint32_t base = sh.base[x] + sh.cnt[x][w];
+    for (uint32_t i = (uint32_t)lane; i < st.cnt; i += kWave) {
+      const uint32_t slot = base + i;
+      if (slot < (uint32_t)p.V)
+        p.fl[x][nxt[x]][slot] = st.buf[i];
+      else
+        atomicOr(&ctl->err, 1u);
+    }
+    st.cnt = 0;
+    HubStage& h = ap.h[x];
+    if (h.cnt) {
+      const unsigned long long hb = sh.hbase[x];
+      hub_write(h, (uint32_t)(hb >> kHubEdgeBits) + sh.cnt[2 + x][w], (hb & kHubEdgeMask) + sh.hedge[x][w],
+                p.hl[x][nxt[x]], p.hb[x][nxt[x]], p.hcap, &ctl->err);
+    }
+    h.cnt = 0;
+  }
 }
 
 // Top-down probe of edge (v -> u) for side x with frontier mask fm, split in
@@ -651,8 +669,14 @@ __device__ __forceinline__ void batch_apply(BatchCtl* ctl, int a, bool allow_fil
     for (int x = 0; x < 2; ++x) {
       if (!(d.sides & (1 << x))) continue;
       const int c = ctl->cur[x];
-      const uint64_t size = (uint64_t)ctl->fl_n[x][c] + (ctl->hub[x][c] >> kHubEdgeBits);
-      if ((ctl->lact[x] & ~ctl->act) && (size >= kFilterMinVertices || ctl->fl_mf[x][c] >= kFilterMinEdges))
+      // expected work saved: the share of the list's queries resolved since
+      // it was built
+      const int built = __popcll(ctl->lact[x]);
+      const int dead = __popcll(ctl->lact[x] & ~ctl->act);
+      if (!dead || !built) continue;
+      const double frac = (double)dead / (double)built;
+      const double size = (double)ctl->fl_n[x][c] + (double)(ctl->hub[x][c] >> kHubEdgeBits);
+      if (size * frac >= (double)kFilterMinVertices || (double)ctl->fl_mf[x][c] * frac >= (double)kFilterMinEdges)
         filt |= 1 << x;
     }
   }
@@ -681,18 +705,36 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
                                               const RedTot& r) {
   BatchCtl* ctl = p.ctl;
   const int wave0 = ctl->wave * 64;
+  // Counters the launch updated with atomics, all loads issued together.
+  uint32_t ln[2] = {0, 0};
+  unsigned long long lh[2] = {0, 0}, lm[2] = {0, 0}, act0 = 0;
+  if (phase == kBInit) {
+    act0 = ld_agent(&ctl->act);
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      ln[x] = ld_agent(&ctl->fl_n[x][0]);
+      lh[x] = ld_agent(&ctl->hub[x][0]);
+      lm[x] = ld_agent(&ctl->fl_mf[x][0]);
+    }
+  } else if (phase == kBCommit || phase == kBBottomUp || phase == kBFilter) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      ln[x] = ld_agent(&ctl->fl_n[x][ctl->cur[x] ^ 1]);
+      lh[x] = ld_agent(&ctl->hub[x][ctl->cur[x] ^ 1]);
+    }
+  }
   ctl->active_vertices += (int64_t)r.v[kRActV];
   ctl->active_edges += (int64_t)r.v[kRActE];
   ctl->touched_total += (int64_t)r.v[kRTouched];
   switch (phase) {
     case kBInit: {
-      const unsigned long long act = ld_agent(&ctl->act);
+      const unsigned long long act = act0;
       ctl->act = act;
       ctl->lvl[0] = ctl->lvl[1] = 0;
       for (int x = 0; x < 2; ++x) {
-        ctl->fl_n[x][0] = ld_agent(&ctl->fl_n[x][0]);
-        ctl->hub[x][0] = ld_agent(&ctl->hub[x][0]);
-        ctl->fl_mf[x][0] = ld_agent(&ctl->fl_mf[x][0]);
+        ctl->fl_n[x][0] = ln[x];
+        ctl->hub[x][0] = lh[x];
+        ctl->fl_mf[x][0] = lm[x];
         ctl->lact[x] = act;
       }
       return batch_next(ctl, false);
@@ -732,8 +774,8 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
         }
         ctl->fl_mf[x][c] = 0;
         ctl->cur[x] = c ^ 1;
-        ctl->fl_n[x][c ^ 1] = ld_agent(&ctl->fl_n[x][c ^ 1]);
-        ctl->hub[x][c ^ 1] = ld_agent(&ctl->hub[x][c ^ 1]);
+        ctl->fl_n[x][c ^ 1] = ln[x];
+        ctl->hub[x][c ^ 1] = lh[x];
         ctl->fl_mf[x][c ^ 1] = r.v[kRMfS + x];
       }
       if (phase == kBBottomUp) {
@@ -760,8 +802,8 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
         ctl->hub[x][c] = 0;
         ctl->fl_mf[x][c] = 0;
         ctl->cur[x] = c ^ 1;
-        ctl->fl_n[x][c ^ 1] = ld_agent(&ctl->fl_n[x][c ^ 1]);
-        ctl->hub[x][c ^ 1] = ld_agent(&ctl->hub[x][c ^ 1]);
+        ctl->fl_n[x][c ^ 1] = ln[x];
+        ctl->hub[x][c ^ 1] = lh[x];
         ctl->fl_mf[x][c ^ 1] = r.v[kRMfS + x];
         ctl->lact[x] = act;
       }
@@ -983,8 +1025,21 @@ __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sh.last = t == nb - 1;
+    // Two-level ticket: the blocks of group b % kRedSlots count on their own
+    // word, and the last block of each group counts on the launch ticket, so
+    // no single address takes an atomic from every block.
+    const uint32_t k = blockIdx.x % kRedSlots;
+    const uint32_t gsize = (nb - k + kRedSlots - 1) / kRedSlots;
+    unsigned long long* sub = p.red + (size_t)(kRedSlots + k) * kRedWords;
+    const unsigned long long t1 = __hip_atomic_fetch_add(sub, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool last = false;
+    if (t1 == gsize - 1) {
+      __hip_atomic_store(sub, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t groups = nb < (uint32_t)kRedSlots ? nb : (uint32_t)kRedSlots;
+      const uint32_t t2 = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = t2 == groups - 1;
+    }
+    sh.last = last;
   }
   __syncthreads();
   if (!sh.last || threadIdx.x >= kWave) return;
@@ -1094,7 +1149,7 @@ static fc_status batch_alloc(fc_graph* g) {
   const size_t o_tm = o;
   o += al((V + 31) / 32 + 64);
   const size_t o_red = o;
-  o += al(sizeof(unsigned long long) * kRedSlots * kRedWords);
+  o += al(sizeof(unsigned long long) * 2 * kRedSlots * kRedWords);  // slots, then group tickets
   const size_t o_fl = o;
   o += 4 * al(4 * V);
   const size_t o_hl = o;
@@ -1231,7 +1286,7 @@ fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile) {
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error
-  FC_HIP(hipMemsetAsync(b->red, 0, sizeof(unsigned long long) * kRedSlots * kRedWords, g->stream));
+  FC_HIP(hipMemsetAsync(b->red, 0, sizeof(unsigned long long) * 2 * kRedSlots * kRedWords, g->stream));
   return FC_OK;
 }
 
