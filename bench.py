@@ -1,14 +1,21 @@
-"""Benchmark of the BFS / s–t connectivity hot path on one MI355X.
+"""Benchmark of the s–t connectivity / BFS hot path on one MI355X.
 
 Workload (one "step"): 1,024 seeded s–t queries on an undirected R-MAT
 scale-22 graph (edgefactor 16, a=0.57 b=0.19 c=0.19 d=0.05, int32 CSR),
-answered by the engine's fastest exact path. Inputs are resident in HBM
-before the timed region. Also measured in the same run: the plain
-one-query-per-launch path on the same pairs, and BFS from 64 seeded roots
-(GTEPS). The CPU baseline is the oracle's single-thread host BFS s–t
-(BFS from s, read the level of t) on a bounded sample of the same pairs.
+answered by the batched bit-parallel engine (64-query waves) under the
+learned scheduler when trained parameters are present, else under the
+heuristic scheduler. `value` is queries per second of device time (HIP
+events around the kernel launches; the query and result arrays, 12 KB, are
+copied outside the timed region), so inputs are resident in HBM.
 
-Prints one JSON line.
+Also measured in the same run, on the same graph:
+  * the plain one-query-per-launch path on the same pairs (q/s and the
+    batched/plain speedup), verdicts and distances compared bit for bit;
+  * BFS from 64 seeded roots (GTEPS and the HBM roofline of the visit kernel);
+  * the CPU baseline: the oracle's single-thread host BFS s–t (BFS from s,
+    level of t) on a bounded sample of the same pairs, also checked.
+
+Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
@@ -40,6 +47,16 @@ def cpu_model():
     return platform.processor()
 
 
+def batch_bytes(stats, V, off64):
+    """Algorithmic bytes of a batched run: per step and active frontier
+    vertex 8 (mask) + o (offset pair) + 8 (visited) + 8 (next mask) bytes, per
+    active edge 4 (column) + 8 (neighbour mask) bytes, plus the four 8-byte
+    mask words per vertex cleared per wave."""
+    o = 16 if off64 else 8
+    return (12 * stats["active_edges"] + (24 + o) * stats["active_vertices"]
+            + 32 * V * stats["waves"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,15 +67,39 @@ def main():
     ap.add_argument("--pairs", type=int, default=1024)
     ap.add_argument("--roots", type=int, default=64)
     ap.add_argument("--seed", type=int, default=22)
+    ap.add_argument("--policy", choices=["auto", "plain", "heuristic", "learned"], default="auto")
+    ap.add_argument("--vw", type=int, default=0, help="virtual-warp width (0: autotuned)")
+    ap.add_argument("--tile", type=int, default=0, help="hub chunk in edges (0: autotuned)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "plain", "bfs"], default="all")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-bfs", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the phases directly instead of replaying captured hipGraphs "
+                         "(for counter-collection runs, which do not support stream capture)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB and exit: a kernel of known "
+                         "traffic that scales hardware byte counters in profiling runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration_mib > 0:
+        # torch is used here only for the calibration kernel, in a process of
+        # its own (the engine is not loaded).
+        import torch
+
+        n = args.copy_calibration_mib << 20
+        src = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n} bytes read and {n} bytes written by one copy kernel")
+        return
 
     import fastcon as fc
 
@@ -73,72 +114,112 @@ def main():
     g = fc.Graph(hg)
     log(f"uploaded: offsets {'int64' if g.offsets64 else 'int32'}")
 
-    result = {}
-    # ---- headline: s–t queries/s over the pairs, K timed steps ----
-    def plain_step():
-        return g.stcon_many(s, t, policy=fc.POLICY_PLAIN)
+    policy_name = args.policy
+    if policy_name == "auto":
+        policy_name = "learned" if fc.load_default_sched_params(g) else "heuristic"
+    elif policy_name == "learned" and not fc.load_default_sched_params(g):
+        log("no trained scheduler parameters found")
+        sys.exit(2)
+    policy = {"plain": fc.POLICY_PLAIN, "heuristic": fc.POLICY_HEURISTIC, "learned": fc.POLICY_LEARNED}[policy_name]
 
-    step_fn = plain_step
-    path = "plain"
-    ref = step_fn()  # also the first warm-up step
+    # ---- headline: batched s–t, K timed steps over the pairs ----
+    def step():
+        return g.stcon_batch(s, t, policy=policy, vw=args.vw, tile=args.tile, use_graph=0 if args.no_graph else 1)
+
+    ref = step()  # first warm-up step; runs the width/tile autotuner unless both are given
     for _ in range(args.warmup - 1):
-        step_fn()
-    ms = []
-    dev_ms = []
+        step()
+    dev_ms, wall_ms = [], []
     stats = None
     for _ in range(args.steps):
         a = time.perf_counter()
-        r = step_fn()
-        ms.append((time.perf_counter() - a) * 1e3)
+        r = step()
+        wall_ms.append((time.perf_counter() - a) * 1e3)
         dev_ms.append(r.stats["time_ms"])
         stats = r.stats
         assert np.array_equal(r.dist, ref.dist)
-    ms_step = float(np.mean(ms))
+    ms_step = float(np.mean(dev_ms))
     qps = args.pairs / (ms_step / 1e3)
-    plain_qps = qps
-    log(f"{path}: {ms_step:.2f} ms/step wall, {np.mean(dev_ms):.2f} ms device, {qps:.0f} q/s")
+    log(f"batched ({policy_name}): {ms_step:.3f} ms/step device, {np.mean(wall_ms):.3f} ms wall, {qps:.0f} q/s; "
+        f"vw={stats['vw']} tile={stats['tile']} launches={stats['launches']} phases={stats['phases']}")
 
-    # Roofline of the dominant kernel (the per-query visit launch):
-    # B_q = 8*E_x + 20*V_x + 4*V algorithmic bytes per launch, E_x and V_x the
-    # expanded edges and vertices reported by the kernel.
-    launches = int(np.count_nonzero(s != t))
-    bytes_total = 8 * stats["expanded_edges"] + 20 * stats["expanded_vertices"] + 4 * V * launches
-    avg_launch_ms = float(np.mean(dev_ms)) / launches
-    achieved = bytes_total / launches / (avg_launch_ms / 1e3) / 1e9
+    # Roofline of the dominant kernel (batch_kernel, every phase of the wave
+    # loop is one launch of it): algorithmic bytes per launch over the mean
+    # launch duration, both from the same timed region.
+    launches = int(stats["launches"])
+    bytes_run = batch_bytes(stats, V, g.offsets64)
+    avg_launch_us = ms_step * 1e3 / launches
+    achieved = bytes_run / (ms_step / 1e3) / 1e9
+    # Measured HBM traffic (FETCH_SIZE and WRITE_SIZE counter runs, scaled by
+    # a copy kernel of known size) is kept per step in profiles/; it applies
+    # when this run is the workload it was measured on.
+    traffic, traffic_src = None, None
+    pmc_path = os.path.join(ROOT, "profiles", "batch_kernel_rmat_s22_pmc.json")
+    if os.path.exists(pmc_path):
+        pmc = json.load(open(pmc_path))
+        m = pmc.get("match", {})
+        if (m.get("scale"), m.get("edgefactor"), m.get("pairs"), m.get("policy"), m.get("seed")) == \
+                (args.scale, args.edgefactor, args.pairs, policy_name, args.seed):
+            traffic = int(pmc["traffic_bytes_per_step"] / launches)
+            traffic_src = pmc_path[len(ROOT) + 1:]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "visit_kernel<int> (plain two-colour s–t, one query per launch)",
-                "bytes_per_launch": int(bytes_total / launches), "avg_launch_us": round(avg_launch_ms * 1e3, 2)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": "fcd::batch_kernel<int>", "bytes_per_launch": int(bytes_run / launches),
+                "avg_launch_us": round(avg_launch_us, 2), "launches_per_step": launches,
+                "working_launches_per_step": int(stats["phases"])}
+
+    # ---- plain one-query-per-launch path on the same pairs ----
+    plain = None
+    if not args.no_plain:
+        g.stcon_many(s[:64], t[:64])
+        pr = g.stcon_many(s, t, policy=fc.POLICY_PLAIN)
+        assert np.array_equal(pr.dist, ref.dist), "batched and plain distances differ"
+        p_ms = pr.stats["time_ms"]
+        plain_qps = args.pairs / (p_ms / 1e3)
+        nq = int(np.count_nonzero(s != t))
+        pst = pr.stats
+        pbytes = 8 * pst["expanded_edges"] + 20 * pst["expanded_vertices"] + 4 * V * nq
+        plain = {"qps": round(plain_qps, 1), "ms": round(p_ms, 3), "speedup_batched": round(qps / plain_qps, 2),
+                 "roofline": {"kernel": "fcd::phase_kernel<int> (one query per launch sequence)",
+                              "achieved": round(pbytes / (p_ms / 1e3) / 1e9, 1), "unit": "GB/s",
+                              "frac": round(pbytes / (p_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        log(f"plain: {p_ms:.1f} ms = {plain_qps:.0f} q/s; batched speedup {qps / plain_qps:.2f}x")
 
     # ---- BFS GTEPS ----
     bfs = None
-    if args.only in ("all", "bfs"):
-        for r in bfs_roots[:2]:
-            g.bfs(int(r), levels=False)
-        e_tot, ms_tot, lv = 0, 0.0, []
-        for r in bfs_roots:
-            _, _, st = g.bfs(int(r), levels=False)
+    if not args.no_bfs:
+        for r0 in bfs_roots[:2]:
+            g.bfs(int(r0), levels=False)
+        e_tot, v_tot, ms_tot, lv = 0, 0, 0.0, []
+        for r0 in bfs_roots:
+            _, _, st = g.bfs(int(r0), levels=False)
             e_tot += st["visited_edges"]
+            v_tot += st["visited_vertices"]
             ms_tot += st["time_ms"]
             lv.append(st["levels"])
         gteps = e_tot / (ms_tot / 1e3) / 1e9
-        bfs_bytes = None
+        o = 28 if g.offsets64 else 20
+        b_bfs = 8 * e_tot + o * v_tot + 4 * V * len(bfs_roots)
+        bw = b_bfs / (ms_tot / 1e3) / 1e9
         bfs = {"roots": len(bfs_roots), "gteps": round(gteps, 2), "gteps_graph500": round(gteps / 2, 2),
-               "avg_ms": round(ms_tot / len(bfs_roots), 3), "avg_levels": float(np.mean(lv))}
-        log(f"bfs: {gteps:.1f} GTEPS over {len(bfs_roots)} roots, {ms_tot / len(bfs_roots):.3f} ms avg")
+               "avg_ms": round(ms_tot / len(bfs_roots), 4), "avg_levels": float(np.mean(lv)),
+               "roofline": {"bound": "hbm", "achieved": round(bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(bw / HBM_PEAK_GBS, 4), "kernel": "fcd::phase_kernel<int> (BFS)"}}
+        log(f"bfs: {gteps:.1f} GTEPS over {len(bfs_roots)} roots, {ms_tot / len(bfs_roots):.3f} ms avg, "
+            f"{bw:.0f} GB/s algorithmic = {bw / HBM_PEAK_GBS:.1%} of HBM peak")
 
     # ---- CPU baseline: oracle host BFS s–t, one thread, bounded sample ----
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        sec1, d1 = orc.time_stcon(hg.offsets, hg.columns, s[:1], t[:1])
+        sec1, _ = orc.time_stcon(hg.offsets, hg.columns, s[:1], t[:1])
         n = int(max(1, min(args.pairs, args.cpu_seconds / max(sec1, 1e-6))))
         sec, dcpu = orc.time_stcon(hg.offsets, hg.columns, s[:n], t[:n])
         assert np.array_equal(dcpu, ref.dist[:n]), "GPU distances differ from the host BFS"
         cpu = {"value": round(n / sec, 3), "unit": "queries/s", "cores": 1, "kind": "port",
                "sample": f"first {n} of the {args.pairs} pairs, host BFS from s then level of t "
-                         f"(oracle restatement of BFS-4K Graph::bfs), {sec:.1f}s",
+                         f"(oracle restatement of the reference host BFS), {sec:.1f}s",
                "cpu": cpu_model(), "nproc": os.cpu_count()}
         log(f"cpu baseline: {n} queries in {sec:.1f}s = {n / sec:.2f} q/s")
 
@@ -149,16 +230,19 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "u64",
         "data": "synthetic (seeded R-MAT generator, seeded pairs among degree>0 vertices)",
-        "config": {"workload": f"{args.pairs} s–t queries on undirected R-MAT s{args.scale} ef{int(args.edgefactor)}",
+        "config": {"workload": f"{args.pairs} s–t queries, batched in 64-query waves, on undirected R-MAT "
+                               f"s{args.scale} ef{int(args.edgefactor)}",
                    "scale": args.scale, "edgefactor": args.edgefactor, "V": V, "E_directed": E,
-                   "pairs": args.pairs, "path": path, "seed": args.seed},
-        "plain_qps": round(plain_qps, 1),
+                   "pairs": args.pairs, "policy": policy_name, "seed": args.seed,
+                   "vw": stats["vw"], "tile": stats["tile"]},
+        "wall_ms_per_step": round(float(np.mean(wall_ms)), 4),
+        "plain": plain,
         "bfs": bfs,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -369,3 +369,19 @@ class Graph:
         _check(_lib.fc_sched_gradient(self._h, x.ctypes.data_as(_f32p), tg.ctypes.data_as(_f32p), len(x),
                                       C.byref(loss), g.ctypes.data_as(_f32p)))
         return loss.value, g
+
+
+# Trained scheduler parameters shipped with the package (written by
+# fastcon/train_sched.py); absent until a training run has been committed.
+DEFAULT_SCHED_PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sched_params.npy")
+
+
+def load_default_sched_params(g: "Graph", path: str = DEFAULT_SCHED_PARAMS) -> bool:
+    """Set the shipped scheduler parameters on g; False when there are none."""
+    if not os.path.exists(path):
+        return False
+    w = np.load(path, allow_pickle=False)
+    if w.dtype != np.float32 or w.shape != (SCHED_PARAMS,):
+        raise ValueError(f"{path}: expected {SCHED_PARAMS} float32 parameters")
+    g.sched_set_params(w)
+    return True
