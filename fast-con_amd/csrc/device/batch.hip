@@ -822,7 +822,8 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
       }
       ctl->act = 0;
       ctl->wave++;
-      ctl->phase = ctl->wave < ctl->nwaves ? kBInit : kBIdle;
+      ctl->t_wave_end = (int64_t)wall_clock64();
+      ctl->phase = ctl->wave < ctl->nwaves && ctl->wave < ctl->stop_wave ? kBInit : kBIdle;
       return kFinDone;
     }
     default:
@@ -897,7 +898,7 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
   BatchCtl* ctl = p.ctl;
   const int phase = ctl->phase;
-  if (phase == kBIdle) return;
+  if (phase == kBIdle || phase == kBPause) return;
   const uint32_t nb = gridDim.x;
   const int gwave = (int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   const int nwaves = (int)(nb * kWavesPerBlock);
@@ -1055,7 +1056,10 @@ __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
         ctl->ptime[ctl->phases] = (int64_t)wall_clock64();
       }
       ctl->phases++;
-      need = batch_finalize(p, phase, d, tot);
+      if (phase == kBDecide)
+        need = ctl->pause_filter ? kFinDecideFilter : kFinDecide;
+      else
+        need = batch_finalize(p, phase, d, tot);
     }
   }
   need = __shfl(need, 0);
@@ -1064,12 +1068,24 @@ __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
     int a = 0;
     if (threadIdx.x == 0) a = batch_heuristic(ctl, p.V, p.E, f);
     a = __shfl(a, 0);
-    if (ctl->policy == 2 && p.mlp) {
+    const bool forced = phase == kBDecide && ctl->force_action >= 0;
+    const bool pause = phase != kBDecide && ctl->trace;
+    if (!forced && !pause && ctl->policy == 2 && p.mlp) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) f[k] = __shfl(f[k], 0);
       a = mlp_argmin_wave(p.mlp, f);
     }
-    if (threadIdx.x == 0) batch_apply(ctl, a, need == kFinDecideFilter);
+    if (threadIdx.x == 0) {
+      if (pause) {
+        ctl->heur_action = a;
+        ctl->pause_filter = need == kFinDecideFilter;
+        ctl->phase = kBPause;
+      } else {
+        if (forced) a = ctl->force_action;
+        batch_apply(ctl, a, need == kFinDecideFilter);
+      }
+      if (phase == kBDecide) ctl->t_decide = (int64_t)wall_clock64();
+    }
   }
   if (threadIdx.x == 0) __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1120,6 +1136,7 @@ struct BatchWorkspace {
   double est_launches = 12.0;
   int32_t* h_dist = nullptr;  // pinned
   int32_t* h_phase = nullptr; // pinned
+  char* snap = nullptr;        // tracing: copy of the workspace
 };
 
 void batch_free(BatchWorkspace* b) {
@@ -1132,6 +1149,7 @@ void batch_free(BatchWorkspace* b) {
   (void)hipFree(b->dist);
   if (b->h_dist) (void)hipHostFree(b->h_dist);
   if (b->h_phase) (void)hipHostFree(b->h_phase);
+  (void)hipFree(b->snap);
   delete b;
 }
 
@@ -1272,7 +1290,7 @@ static fc_status batch_graph(fc_graph* g) {
 }
 
 // Prepare the control block for a batch of n queries (already on the device).
-fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile) {
+fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile, int trace) {
   BatchWorkspace* b = g->batch;
   BatchCtl init;
   std::memset(&init, 0, sizeof(init));
@@ -1283,6 +1301,9 @@ fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile) {
   init.policy = policy;
   init.vw_fixed = vw;
   init.tile = tile;
+  init.force_action = -1;
+  init.stop_wave = init.nwaves;
+  init.trace = trace;
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error
@@ -1336,7 +1357,7 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
   }
   FC_HIP(hipMemcpyAsync(b->qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
-  st = batch_begin(g, n, policy, vw, tile);
+  st = batch_begin(g, n, policy, vw, tile, 0);
   if (st != FC_OK) return st;
   FC_HIP(hipStreamSynchronize(g->stream));
   st = batch_drive(g, n, use_graph, ms, launches);
@@ -1382,6 +1403,33 @@ static fc_status batch_autotune(fc_graph* g, const int32_t* s, const int32_t* t,
     }
   b->tuned_vw = bvw;
   b->tuned_tile = btile;
+  return FC_OK;
+}
+
+// Tracing: run direct launches until the phase machine pauses or goes idle.
+static fc_status drive_to_stop(fc_graph* g, int64_t cap) {
+  BatchWorkspace* b = g->batch;
+  for (int64_t launched = 0;; launched += 8) {
+    for (int i = 0; i < 8; ++i) {
+      hipError_t e = batch_launch_raw(g);
+      if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+    }
+    FC_HIP(hipMemcpyAsync(b->h_phase, b->ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    FC_HIP(hipStreamSynchronize(g->stream));
+    if (b->h_phase[0] == kBIdle || b->h_phase[0] == kBPause) return FC_OK;
+    if (launched > cap) return fail(FC_ERR_CAPACITY, "batch phase machine did not stop after %lld launches", (long long)launched);
+  }
+}
+
+static fc_status put_ctl(fc_graph* g, const BatchCtl& c) {
+  FC_HIP(hipMemcpyAsync(g->batch->ctl, &c, sizeof(BatchCtl), hipMemcpyHostToDevice, g->stream));
+  FC_HIP(hipStreamSynchronize(g->stream));
+  return FC_OK;
+}
+
+static fc_status get_ctl(fc_graph* g, BatchCtl* c) {
+  FC_HIP(hipMemcpyAsync(c, g->batch->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, g->stream));
+  FC_HIP(hipStreamSynchronize(g->stream));
   return FC_OK;
 }
 
@@ -1449,6 +1497,82 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
     stats->filters = (int32_t)c.filters;
     stats->phases = (int32_t)c.phases;
     stats->launches = launches;
+  }
+  return FC_OK;
+}
+
+// Trace of the heuristic schedule: at every step decision the workspace is
+// copied, each action is taken from that state and the rest of the wave run
+// (heuristic schedule), and the device clock gives the time from the
+// decision to the end of the wave; then the copy is restored and the
+// heuristic's own action taken.
+extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, fc_trace_row* rows,
+                                    int32_t cap, int32_t* n_rows, int32_t* dist_out) {
+  fc::clear_error();
+  if (!g || !n_rows || (n > 0 && (!s || !t)) || (cap > 0 && !rows))
+    return fc::fail(FC_ERR_INVALID, "fc_sched_trace: null argument");
+  if (n < 0 || cap < 0) return fc::fail(FC_ERR_INVALID, "fc_sched_trace: negative count");
+  *n_rows = 0;
+  for (int32_t i = 0; i < n; ++i)
+    if (s[i] < 0 || s[i] >= g->V || t[i] < 0 || t[i] >= g->V)
+      return fc::fail(FC_ERR_INVALID, "fc_sched_trace: query %d vertex out of range", i);
+  if (n == 0) return FC_OK;
+  fc_status st = fcd::batch_alloc(g);
+  if (st != FC_OK) return st;
+  g->levels_valid = g->parents_valid = false;
+  fcd::BatchWorkspace* b = g->batch;
+  if ((st = fcd::batch_queries(g, n)) != FC_OK) return st;
+  if (!b->snap && hipMalloc((void**)&b->snap, b->bytes) != hipSuccess) {
+    b->snap = nullptr;
+    return fc::fail(FC_ERR_NOMEM, "fc_sched_trace: no room for the workspace copy (%zu bytes)", b->bytes);
+  }
+  int khz = 0;
+  FC_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device));
+  if (khz <= 0) return fc::fail(FC_ERR_HIP, "fc_sched_trace: device clock rate unknown");
+  FC_HIP(hipMemcpyAsync(b->qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+  FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+  const int vw = b->tuned_vw >= 0 ? b->tuned_vw : 0;
+  const int tile = b->tuned_tile > 0 ? b->tuned_tile : 256;
+  if ((st = fcd::batch_begin(g, n, FC_POLICY_HEURISTIC, vw, tile, 1)) != FC_OK) return st;
+  const int64_t cap_launches = ((int64_t)n / 64 + 1) * (4 * g->V + 16) + 64;
+  std::vector<BatchCtl> cv(2);
+  BatchCtl& c = cv[0];
+  BatchCtl& r = cv[1];
+  for (;;) {
+    if ((st = fcd::drive_to_stop(g, cap_launches)) != FC_OK) return st;
+    if ((st = fcd::get_ctl(g, &c)) != FC_OK) return st;
+    if (c.err) return fc::fail(FC_ERR_CAPACITY, "fc_sched_trace: list guard tripped (err=%u)", c.err);
+    if (c.phase == fcd::kBIdle) break;
+    fc_trace_row row;
+    std::memset(&row, 0, sizeof(row));
+    for (int k = 0; k < FC_SCHED_FEATURES; ++k) row.features[k] = c.feat[k];
+    row.heuristic = c.heur_action;
+    row.wave = c.wave;
+    FC_HIP(hipMemcpyAsync(b->snap, b->mem, b->bytes, hipMemcpyDeviceToDevice, g->stream));
+    for (int a = 0; a < FC_SCHED_ACTIONS; ++a) {
+      if (a > 0) FC_HIP(hipMemcpyAsync(b->mem, b->snap, b->bytes, hipMemcpyDeviceToDevice, g->stream));
+      r = c;
+      r.phase = fcd::kBDecide;
+      r.force_action = a;
+      r.trace = 0;
+      r.stop_wave = c.wave + 1;
+      if ((st = fcd::put_ctl(g, r)) != FC_OK) return st;
+      if ((st = fcd::drive_to_stop(g, cap_launches)) != FC_OK) return st;
+      if ((st = fcd::get_ctl(g, &r)) != FC_OK) return st;
+      if (r.err) return fc::fail(FC_ERR_CAPACITY, "fc_sched_trace: list guard tripped (err=%u)", r.err);
+      row.cost_us[a] = (float)((double)(r.t_wave_end - r.t_decide) * 1000.0 / (double)khz);
+    }
+    FC_HIP(hipMemcpyAsync(b->mem, b->snap, b->bytes, hipMemcpyDeviceToDevice, g->stream));
+    r = c;
+    r.phase = fcd::kBDecide;
+    r.force_action = -1;
+    r.trace = 1;
+    if ((st = fcd::put_ctl(g, r)) != FC_OK) return st;
+    if (*n_rows < cap) rows[(*n_rows)++] = row;
+  }
+  if (dist_out) {
+    FC_HIP(hipMemcpyAsync(dist_out, b->dist, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
+    FC_HIP(hipStreamSynchronize(g->stream));
   }
   return FC_OK;
 }

@@ -16,7 +16,9 @@ enum BatchPhase : int32_t {
   kBBottomUp = 4,  // bottom-up step (expand + commit in one pass)
   kBClearF = 5,    // clear the consumed frontier masks after a bottom-up step
   kBCleanup = 6,   // zero every mask the wave touched, write results
-  kBFilter = 7     // drop resolved queries from the current frontier lists
+  kBFilter = 7,    // drop resolved queries from the current frontier lists
+  kBPause = 8,     // tracing: stopped at a step decision (launches are no-ops)
+  kBDecide = 9     // tracing: take the pending decision (forced or by policy)
 };
 
 enum BatchSides : int32_t { kSideS = 1, kSideT = 2, kSideBoth = 3 };
@@ -53,6 +55,13 @@ struct BatchCtl {
   int32_t cur[2];          // parity of the current frontier list/mask per side
   int32_t bu_sides;        // sides of the last bottom-up step (for kBClearF)
   int32_t filt;            // sides (bit x) compacted by kBFilter
+  int32_t trace;           // pause at every step decision
+  int32_t force_action;    // kBDecide: action to take, -1 = the policy's
+  int32_t stop_wave;       // go idle when the wave index reaches this
+  int32_t pause_filter;    // the paused decision may filter first
+  int32_t heur_action;     // the heuristic's action at the pause
+  int64_t t_decide;        // device clock at the last kBDecide
+  int64_t t_wave_end;      // device clock at the last wave cleanup
   unsigned long long lact[2];  // live queries when side x's current lists were built
   BatchDecision dec;
   unsigned long long act;  // unresolved queries of the wave

@@ -75,7 +75,8 @@ class BatchStats(C.Structure):
 
 
 class TraceRow(C.Structure):
-    _fields_ = [("features", C.c_float * SCHED_FEATURES), ("cost_us", C.c_float * SCHED_ACTIONS)]
+    _fields_ = [("features", C.c_float * SCHED_FEATURES), ("cost_us", C.c_float * SCHED_ACTIONS),
+                ("heuristic", C.c_int32), ("wave", C.c_int32)]
 
 
 def _sig(name, res, *args):
@@ -110,7 +111,7 @@ _sig("fc_stcon_sched_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_in
      C.POINTER(Stats))
 _sig("fc_stcon_batch", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(BatchOpts),
      C.POINTER(BatchStats))
-_sig("fc_sched_trace", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.POINTER(TraceRow), C.c_int32, _i32p)
+_sig("fc_sched_trace", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.POINTER(TraceRow), C.c_int32, _i32p, _i32p)
 _sig("fc_sched_train", C.c_int, C.c_void_p, C.POINTER(TraceRow), C.c_int32, C.c_int32, C.c_float, C.c_uint64,
      _f32p)
 _sig("fc_sched_set_params", C.c_int, C.c_void_p, _f32p, C.c_int32)
@@ -320,30 +321,44 @@ class Graph:
         return StResult(conn, dist, st.as_dict())
 
     # ---- learned scheduler ----
-    def sched_trace(self, s, t, cap: int = 4096) -> np.ndarray:
+    def sched_trace(self, s, t, cap: int = 4096):
+        """Trace the heuristic schedule over the queries. Returns (features
+        [rows, 12], cost_us [rows, 6], heuristic action [rows], wave [rows],
+        distances [n])."""
         s = np.ascontiguousarray(s, np.int32)
         t = np.ascontiguousarray(t, np.int32)
-        rows = (TraceRow * cap)()
+        rows = (TraceRow * max(cap, 1))()
         n = C.c_int32()
-        _check(_lib.fc_sched_trace(self._h, _p32(s), _p32(t), len(s), rows, cap, C.byref(n)))
-        out = np.zeros((n.value, SCHED_FEATURES + SCHED_ACTIONS), np.float32)
-        for i in range(n.value):
-            out[i, :SCHED_FEATURES] = rows[i].features[:]
-            out[i, SCHED_FEATURES:] = rows[i].cost_us[:]
-        return out
+        dist = np.empty(len(s), np.int32)
+        _check(_lib.fc_sched_trace(self._h, _p32(s), _p32(t), len(s), rows, cap, C.byref(n), _p32(dist)))
+        k = n.value
+        arr = np.ctypeslib.as_array(rows)[:k] if k else None
+        feats = np.zeros((k, SCHED_FEATURES), np.float32)
+        cost = np.zeros((k, SCHED_ACTIONS), np.float32)
+        heur = np.zeros(k, np.int32)
+        wave = np.zeros(k, np.int32)
+        for i in range(k):
+            feats[i] = rows[i].features[:]
+            cost[i] = rows[i].cost_us[:]
+            heur[i] = rows[i].heuristic
+            wave[i] = rows[i].wave
+        return feats, cost, heur, wave, dist
 
     @staticmethod
-    def _rows(table: np.ndarray):
-        rows = (TraceRow * len(table))()
-        for i, r in enumerate(table):
-            rows[i].features[:] = [float(x) for x in r[:SCHED_FEATURES]]
-            rows[i].cost_us[:] = [float(x) for x in r[SCHED_FEATURES:]]
+    def _rows(feats: np.ndarray, cost: np.ndarray):
+        rows = (TraceRow * len(feats))()
+        for i in range(len(feats)):
+            rows[i].features[:] = [float(x) for x in feats[i]]
+            rows[i].cost_us[:] = [float(x) for x in cost[i]]
         return rows
 
-    def sched_train(self, table: np.ndarray, epochs: int = 200, lr: float = 1e-2, seed: int = 1) -> float:
-        rows = self._rows(table)
+    def sched_train(self, feats: np.ndarray, cost: np.ndarray, epochs: int = 2000, lr: float = 3e-3,
+                    seed: int = 1) -> float:
+        """Train the scheduler network on the device from traced rows; the
+        trained parameters are set on this graph. Returns the final loss."""
+        rows = self._rows(feats, cost)
         loss = C.c_float()
-        _check(_lib.fc_sched_train(self._h, rows, len(table), epochs, lr, seed, C.byref(loss)))
+        _check(_lib.fc_sched_train(self._h, rows, len(feats), epochs, lr, seed, C.byref(loss)))
         return loss.value
 
     def sched_set_params(self, w: np.ndarray):

@@ -237,18 +237,34 @@ fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t* t, int32_
 #define FC_SCHED_ACTIONS 6
 #define FC_SCHED_PARAMS (12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6)
 
-/* Per-step trace record logged by timing every alternative from one state. */
+/* Scheduler actions of a batched step: sides {s, t, both}[a % 3], top-down
+ * for a < 3, bottom-up for a >= 3. */
+#define FC_ACTION_S_TOPDOWN 0
+#define FC_ACTION_T_TOPDOWN 1
+#define FC_ACTION_BOTH_TOPDOWN 2
+#define FC_ACTION_S_BOTTOMUP 3
+#define FC_ACTION_T_BOTTOMUP 4
+#define FC_ACTION_BOTH_BOTTOMUP 5
+
+/* Per-step trace record logged by timing every alternative from one state:
+ * cost_us[a] is the device time from taking action a at this step to the
+ * end of the wave, the rest of the wave following the heuristic. */
 typedef struct fc_trace_row {
   float features[FC_SCHED_FEATURES];
-  float cost_us[FC_SCHED_ACTIONS];   /* measured time of each alternative, microseconds */
+  float cost_us[FC_SCHED_ACTIONS];   /* microseconds */
+  int32_t heuristic;                 /* the heuristic's action at this step */
+  int32_t wave;                      /* wave index within the traced batch */
 } fc_trace_row;
 
-/* Run queries step by step, timing all FC_SCHED_ACTIONS alternatives at each
- * step, and append up to cap rows. Returns the number written in *n_rows. */
+/* Run the batched engine along the heuristic schedule, pausing at every
+ * step decision to time all FC_SCHED_ACTIONS alternatives (device state is
+ * snapshot and restored around each), and append up to cap rows. Returns the
+ * number written in *n_rows; dist_out (nullable) receives the distances. */
 fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
-                         fc_trace_row* rows, int32_t cap, int32_t* n_rows);
-/* Train on the device (hand-written forward/backward/Adam kernels). Loss is
- * the mean squared error of log-cost predictions. */
+                         fc_trace_row* rows, int32_t cap, int32_t* n_rows, int32_t* dist_out);
+/* Train on the device (hand-written forward/backward/Adam kernels), full
+ * batch, from a seeded initialisation. Targets are log(cost_us) minus the
+ * row's mean over the actions; the loss is 0.5*sum((y-target)^2)/n_rows. */
 fc_status fc_sched_train(fc_graph* g, const fc_trace_row* rows, int32_t n_rows, int32_t epochs,
                          float lr, uint64_t seed, float* final_loss);
 fc_status fc_sched_set_params(fc_graph* g, const float* params, int32_t n);

@@ -1,0 +1,105 @@
+"""GPU kernels of the learned scheduler against the CPU oracle (forward,
+gradient, one training step), the trace of the batched engine (answers
+unchanged, costs positive, heuristic action recorded), and what a trained
+network gains over the schedules without it."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_dist(orc, hg, s, t):
+    return np.array([orc.stcon(hg.offsets, hg.columns, a, b)[1] for a, b in zip(s, t)], np.int32)
+
+
+def _data(seed, n=300):
+    rng = np.random.default_rng(seed)
+    w = (rng.normal(0, 0.3, mlp.P)).astype(np.float32)
+    X = rng.uniform(0, 1, (n, mlp.F)).astype(np.float32)
+    T = rng.normal(0, 1, (n, mlp.A)).astype(np.float32)
+    return w, X, T
+
+
+def test_forward_and_gradient_match_oracle(fc):
+    hg = fc.gen_grid(8, 8, 0.0, 1)
+    with fc.Graph(hg) as g:
+        for seed in (1, 2):
+            w, X, T = _data(seed)
+            g.sched_set_params(w)
+            say(f"sched forward/gradient seed={seed}")
+            y = g.sched_forward(X)
+            yo = mlp.forward(w, X)
+            assert np.allclose(y, yo, rtol=1e-5, atol=1e-5)
+            loss, grad = g.sched_gradient(X, T)
+            lo, go = mlp.loss_grad(w, X, T)
+            assert abs(loss - lo) <= 1e-5 * max(1.0, lo)
+            scale = np.abs(go).max()
+            assert np.abs(grad - go).max() <= 1e-4 * scale, np.abs(grad - go).max() / scale
+
+
+def test_one_training_step_matches_oracle(fc):
+    """fc_sched_train with one epoch = seeded init + one Adam step on the
+    full batch; the result equals the oracle's."""
+    hg = fc.gen_grid(8, 8, 0.0, 1)
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0, 1, (200, mlp.F)).astype(np.float32)
+    cost = np.exp(rng.normal(3, 1, (200, mlp.A))).astype(np.float32)
+    T = mlp.row_targets(cost)
+    w0 = mlp.init_params(11).astype(np.float64)
+    _, g0 = mlp.loss_grad(w0, X, T)
+    w1, _, _ = mlp.adam_step(w0, g0, np.zeros(mlp.P), np.zeros(mlp.P), 1, 1e-3)
+    with fc.Graph(hg) as g:
+        say("sched train one epoch")
+        g.sched_train(X, cost, epochs=1, lr=1e-3, seed=11)
+        w = g.sched_get_params()
+    # Adam's first step is lr*sign(g): compare wherever g is not so small
+    # that float32 rounding could flip its sign
+    big = np.abs(g0) > 1e-4 * np.abs(g0).max()
+    close = np.abs(w - w1) <= 2e-6
+    assert close[big].all()
+    assert close.mean() > 0.95
+
+
+def test_training_reduces_loss(fc):
+    hg = fc.gen_grid(8, 8, 0.0, 1)
+    rng = np.random.default_rng(8)
+    X = rng.uniform(0, 1, (400, mlp.F)).astype(np.float32)
+    # costs that depend on the features: action a is cheap when x[a] is small
+    cost = np.exp(2.0 * X[:, :mlp.A] + 0.1 * rng.normal(size=(400, mlp.A))).astype(np.float32)
+    T = mlp.row_targets(cost)
+    with fc.Graph(hg) as g:
+        l1 = g.sched_train(X, cost, epochs=1, lr=3e-3, seed=2)
+        l2 = g.sched_train(X, cost, epochs=400, lr=3e-3, seed=2)
+        w = g.sched_get_params()
+    assert l2 < 0.25 * l1
+    lo, _ = mlp.loss_grad(w.astype(np.float64), X, T)
+    assert abs(lo - l2) <= 1e-3 * max(1.0, lo)
+    pick = mlp.argmin_actions(w, X)
+    assert (pick == np.argmin(cost, 1)).mean() > 0.8
+
+
+def test_trace_keeps_answers_and_times_every_action(fc, orc):
+    hg = fc.gen_rmat(14, 16, seed=3)
+    r = hg.sample_roots(256, 5)
+    s, t = r[:128], r[128:]
+    exp = _oracle_dist(orc, hg, s, t)
+    with fc.Graph(hg) as g:
+        say("sched trace rmat14")
+        feats, cost, heur, wave, dist = g.sched_trace(s, t, cap=4096)
+        assert np.array_equal(dist, exp)
+        assert len(feats) >= 2 and set(wave) == {0, 1}
+        assert (cost > 0).all() and np.isfinite(cost).all()
+        assert ((heur >= 0) & (heur < mlp.A)).all()
+        assert np.isfinite(feats).all()
+        # the traced batch still answers correctly afterwards, under every policy
+        for pol in (0, 1):
+            assert np.array_equal(g.stcon_batch(s, t, policy=pol).dist, exp)
