@@ -40,6 +40,7 @@ namespace fcd {
 using fc::fail;
 
 constexpr uint32_t kBStage = 512;  // staged entries per wave and list
+constexpr float kDefaultLearnMargin = 0.1f;
 // A filter launch pays when the expected number of dropped list entries or
 // frontier edges reaches these.
 constexpr uint64_t kFilterMinVertices = 1u << 15;
@@ -458,6 +459,8 @@ __device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends&
       if (nS | nT) deg = bdeg(p, u);
       if (nS) red[kRMfS] += (unsigned long long)deg;
       if (nT) red[kRMfT] += (unsigned long long)deg;
+      red[kRVisS] += (unsigned long long)__popcll(nS);
+      red[kRVisT] += (unsigned long long)__popcll(nT);
     }
     push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
     push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
@@ -527,6 +530,8 @@ __device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& a
       deg = (int64_t)(e - b);
       if (nS) red[kRMfS] += (unsigned long long)deg;
       if (nT) red[kRMfT] += (unsigned long long)deg;
+      red[kRVisS] += (unsigned long long)__popcll(nS);
+      red[kRVisT] += (unsigned long long)__popcll(nT);
     }
     if (DS) push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
     if (DT) push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
@@ -572,36 +577,37 @@ __device__ __forceinline__ void filter_side(const BatchParams<OffT>& p, Appends&
   }
 }
 
-// Features of the pending step for the learned scheduler (12 values, each
-// scaled to about [0, 1]): per side the frontier size and degree sum over
-// the live queries, their mean degree and the side's level; the graph size;
-// the live-query fraction and the hub share of the frontiers.
+// Features of the pending step for the learned scheduler (12 values in
+// about [0, 1], scale-free except f4): per side the live frontier's size and
+// degree sum relative to the graph's (on a log scale), its mean degree
+// relative to the graph's, the side's level and the fraction of (vertex,
+// query) pairs it has visited; the graph's edge count; the live-query
+// fraction.
 __device__ __forceinline__ void batch_features(BatchCtl* ctl, const uint64_t* mf, int64_t V, int64_t E) {
-  auto lg = [](double x) { return (float)log2(x + 1.0); };
+  auto lg = [](double x) { return log2(x + 1.0); };
   float* f = ctl->feat;
   const int built[2] = {(int)__popcll(ctl->lact[0]), (int)__popcll(ctl->lact[1])};
   const int live[2] = {(int)__popcll(ctl->lact[0] & ctl->act), (int)__popcll(ctl->lact[1] & ctl->act)};
   double n[2];
-  uint64_t hn = 0;
   for (int x = 0; x < 2; ++x) {
     const int c = ctl->cur[x];
-    hn += ctl->hub[x][c] >> kHubEdgeBits;
     n[x] = (double)ctl->fl_n[x][c] + (double)(ctl->hub[x][c] >> kHubEdgeBits);
     if (built[x] > 0) n[x] *= (double)live[x] / (double)built[x];
   }
-  f[0] = lg(n[0]) / 32.f;
-  f[1] = lg((double)mf[0]) / 40.f;
-  f[2] = lg(n[1]) / 32.f;
-  f[3] = lg((double)mf[1]) / 40.f;
-  f[4] = lg((double)E) / 40.f;
-  f[5] = lg((double)V) / 32.f;
+  const double pairs = (double)(ctl->live0 > 0 ? ctl->live0 : 1) * (double)V;
+  const double lv = lg((double)V), le = lg((double)E), avg = (double)E / (double)(V > 0 ? V : 1);
+  f[0] = (float)(lg(n[0]) / lv);
+  f[1] = (float)(lg((double)mf[0]) / le);
+  f[2] = (float)(lg(n[1]) / lv);
+  f[3] = (float)(lg((double)mf[1]) / le);
+  f[4] = (float)(le / 40.0);
+  f[5] = (float)((double)ctl->vis_pairs[0] / pairs);
   f[6] = (float)(ctl->lvl[0] < 64 ? ctl->lvl[0] : 64) / 64.f;
   f[7] = (float)(ctl->lvl[1] < 64 ? ctl->lvl[1] : 64) / 64.f;
-  f[8] = lg(n[0] > 0 ? (double)mf[0] / n[0] : 0.0) / 20.f;
-  f[9] = lg(n[1] > 0 ? (double)mf[1] / n[1] : 0.0) / 20.f;
+  f[8] = (float)(lg(n[0] > 0 ? (double)mf[0] / n[0] / avg : 0.0) / 16.0);
+  f[9] = (float)(lg(n[1] > 0 ? (double)mf[1] / n[1] / avg : 0.0) / 16.0);
   f[10] = (float)__popcll(ctl->act) / 64.f;
-  const double ntot = (double)ctl->fl_n[0][ctl->cur[0]] + (double)ctl->fl_n[1][ctl->cur[1]] + (double)hn;
-  f[11] = (float)((double)hn / (ntot + 1.0));
+  f[11] = (float)((double)ctl->vis_pairs[1] / pairs);
 }
 
 __device__ __forceinline__ int bdegree_width(uint64_t n, uint64_t mf) {
@@ -619,7 +625,9 @@ __device__ __forceinline__ void batch_apply_action(BatchDecision& d, int a) {
   d.action = a;
 }
 
-constexpr uint64_t kBothMaxEdges = 1u << 15;  // heuristic: expand both sides while both frontiers are this small
+// Heuristic: expand both sides while the two frontiers together have at most
+// this many edges, about the work an extra step's fixed cost would buy.
+constexpr uint64_t kBothMaxEdges = 1u << 20;
 
 // Degree sum of side x's current frontier restricted to the live queries,
 // estimated from the fraction of the queries the lists were built for that
@@ -726,11 +734,15 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
   ctl->active_vertices += (int64_t)r.v[kRActV];
   ctl->active_edges += (int64_t)r.v[kRActE];
   ctl->touched_total += (int64_t)r.v[kRTouched];
+  ctl->vis_pairs[0] += (int64_t)r.v[kRVisS];
+  ctl->vis_pairs[1] += (int64_t)r.v[kRVisT];
   switch (phase) {
     case kBInit: {
       const unsigned long long act = act0;
       ctl->act = act;
       ctl->lvl[0] = ctl->lvl[1] = 0;
+      ctl->live0 = __popcll(act);
+      ctl->vis_pairs[0] = ctl->vis_pairs[1] = ctl->live0;
       for (int x = 0; x < 2; ++x) {
         ctl->fl_n[x][0] = ln[x];
         ctl->hub[x][0] = lh[x];
@@ -1069,11 +1081,12 @@ __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
     if (threadIdx.x == 0) a = batch_heuristic(ctl, p.V, p.E, f);
     a = __shfl(a, 0);
     const bool forced = phase == kBDecide && ctl->force_action >= 0;
-    const bool pause = phase != kBDecide && ctl->trace;
+    const bool pause = phase != kBDecide && ctl->trace &&
+                       ctl->decisions % (ctl->trace_every > 0 ? ctl->trace_every : 1) == 0;
     if (!forced && !pause && ctl->policy == 2 && p.mlp) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) f[k] = __shfl(f[k], 0);
-      a = mlp_argmin_wave(p.mlp, f);
+      a = mlp_pick_wave(p.mlp, f, a, 1e-3f * (float)ctl->learn_margin_milli);
     }
     if (threadIdx.x == 0) {
       if (pause) {
@@ -1083,6 +1096,7 @@ __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
       } else {
         if (forced) a = ctl->force_action;
         batch_apply(ctl, a, need == kFinDecideFilter);
+        ctl->decisions++;
       }
       if (phase == kBDecide) ctl->t_decide = (int64_t)wall_clock64();
     }
@@ -1290,7 +1304,7 @@ static fc_status batch_graph(fc_graph* g) {
 }
 
 // Prepare the control block for a batch of n queries (already on the device).
-fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile, int trace) {
+fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile, int trace, int every, float margin) {
   BatchWorkspace* b = g->batch;
   BatchCtl init;
   std::memset(&init, 0, sizeof(init));
@@ -1304,6 +1318,8 @@ fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile, int 
   init.force_action = -1;
   init.stop_wave = init.nwaves;
   init.trace = trace;
+  init.trace_every = every;
+  init.learn_margin_milli = (int32_t)std::lround((margin < 0.f ? kDefaultLearnMargin : margin) * 1000.f);
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error
@@ -1347,7 +1363,8 @@ static fc_status batch_drive(fc_graph* g, int32_t n, bool use_graph, float* ms, 
 }
 
 static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy, int vw, int tile,
-                           bool use_graph, float* ms, BatchCtl* ctl_out, int64_t* launches = nullptr) {
+                           bool use_graph, float* ms, BatchCtl* ctl_out, int64_t* launches = nullptr,
+                           float margin = -1.f) {
   BatchWorkspace* b = g->batch;
   fc_status st = batch_queries(g, n);
   if (st != FC_OK) return st;
@@ -1357,7 +1374,7 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
   }
   FC_HIP(hipMemcpyAsync(b->qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
-  st = batch_begin(g, n, policy, vw, tile, 0);
+  st = batch_begin(g, n, policy, vw, tile, 0, 1, margin);
   if (st != FC_OK) return st;
   FC_HIP(hipStreamSynchronize(g->stream));
   st = batch_drive(g, n, use_graph, ms, launches);
@@ -1478,7 +1495,7 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
   float ms = 0;
   BatchCtl c;
   int64_t launches = 0;
-  st = fcd::batch_run(g, s, t, n, o.policy, vw, tile, o.use_graph != 0, &ms, &c, &launches);
+  st = fcd::batch_run(g, s, t, n, o.policy, vw, tile, o.use_graph != 0, &ms, &c, &launches, o.learn_margin);
   if (st != FC_OK) return st;
   for (int32_t i = 0; i < n; ++i) {
     const int32_t d = b->h_dist[i];
@@ -1506,12 +1523,17 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
 // (heuristic schedule), and the device clock gives the time from the
 // decision to the end of the wave; then the copy is restored and the
 // heuristic's own action taken.
-extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, fc_trace_row* rows,
-                                    int32_t cap, int32_t* n_rows, int32_t* dist_out) {
+extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy,
+                                    int32_t every,
+                                    fc_trace_row* rows, int32_t cap, int32_t* n_rows, int32_t* dist_out) {
   fc::clear_error();
   if (!g || !n_rows || (n > 0 && (!s || !t)) || (cap > 0 && !rows))
     return fc::fail(FC_ERR_INVALID, "fc_sched_trace: null argument");
-  if (n < 0 || cap < 0) return fc::fail(FC_ERR_INVALID, "fc_sched_trace: negative count");
+  if (n < 0 || cap < 0 || every < 1) return fc::fail(FC_ERR_INVALID, "fc_sched_trace: bad count or stride");
+  if (policy != FC_POLICY_HEURISTIC && policy != FC_POLICY_LEARNED)
+    return fc::fail(FC_ERR_INVALID, "fc_sched_trace: policy must be heuristic or learned");
+  if (policy == FC_POLICY_LEARNED && !g->mlp_set)
+    return fc::fail(FC_ERR_INVALID, "fc_sched_trace: learned policy requested but no scheduler parameters are set");
   *n_rows = 0;
   for (int32_t i = 0; i < n; ++i)
     if (s[i] < 0 || s[i] >= g->V || t[i] < 0 || t[i] >= g->V)
@@ -1533,7 +1555,7 @@ extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t
   FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   const int vw = b->tuned_vw >= 0 ? b->tuned_vw : 0;
   const int tile = b->tuned_tile > 0 ? b->tuned_tile : 256;
-  if ((st = fcd::batch_begin(g, n, FC_POLICY_HEURISTIC, vw, tile, 1)) != FC_OK) return st;
+  if ((st = fcd::batch_begin(g, n, policy, vw, tile, cap > 0 ? 1 : 0, every, 0.f)) != FC_OK) return st;
   const int64_t cap_launches = ((int64_t)n / 64 + 1) * (4 * g->V + 16) + 64;
   std::vector<BatchCtl> cv(2);
   BatchCtl& c = cv[0];
@@ -1563,12 +1585,12 @@ extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t
       row.cost_us[a] = (float)((double)(r.t_wave_end - r.t_decide) * 1000.0 / (double)khz);
     }
     FC_HIP(hipMemcpyAsync(b->mem, b->snap, b->bytes, hipMemcpyDeviceToDevice, g->stream));
+    if (*n_rows < cap) rows[(*n_rows)++] = row;
     r = c;
     r.phase = fcd::kBDecide;
     r.force_action = -1;
-    r.trace = 1;
+    r.trace = *n_rows < cap ? 1 : 0;  // full: run the rest untraced
     if ((st = fcd::put_ctl(g, r)) != FC_OK) return st;
-    if (*n_rows < cap) rows[(*n_rows)++] = row;
   }
   if (dist_out) {
     FC_HIP(hipMemcpyAsync(dist_out, b->dist, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, g->stream));

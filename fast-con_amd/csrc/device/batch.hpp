@@ -39,7 +39,9 @@ constexpr int kRedSlots = 16;
 constexpr int kRedWords = 16;  // words per slot (one 128-byte line)
 enum BatchRed : int32_t {
   kROdd = 0, kREven, kRAnyS, kRAnyT,  // OR
-  kRMfS, kRMfT, kRActV, kRActE, kRTouched, kRCount  // sum
+  kRMfS, kRMfT, kRActV, kRActE, kRTouched,
+  kRVisS, kRVisT,  // (vertex, query) pairs newly visited per side
+  kRCount  // sum
 };
 
 struct BatchCtl {
@@ -55,11 +57,19 @@ struct BatchCtl {
   int32_t cur[2];          // parity of the current frontier list/mask per side
   int32_t bu_sides;        // sides of the last bottom-up step (for kBClearF)
   int32_t filt;            // sides (bit x) compacted by kBFilter
-  int32_t trace;           // pause at every step decision
+  int32_t trace;           // pause at step decisions (every trace_every-th one)
+  int32_t trace_every;
+  int32_t decisions;       // step decisions taken so far
   int32_t force_action;    // kBDecide: action to take, -1 = the policy's
   int32_t stop_wave;       // go idle when the wave index reaches this
   int32_t pause_filter;    // the paused decision may filter first
   int32_t heur_action;     // the heuristic's action at the pause
+  int32_t learn_margin_milli;  // learned policy: keep the heuristic's action unless the
+                               // network predicts a log-cost lower by this (x1e-3)
+  int32_t pad0;
+  int64_t vis_pairs[2];    // (vertex, query) pairs visited per side in this wave
+  int32_t live0;           // live queries at the start of the wave
+  int32_t pad1;
   int64_t t_decide;        // device clock at the last kBDecide
   int64_t t_wave_end;      // device clock at the last wave cleanup
   unsigned long long lact[2];  // live queries when side x's current lists were built

@@ -61,7 +61,7 @@ class Stats(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("policy", C.c_int32), ("vw", C.c_int32), ("tile", C.c_int32), ("use_graph", C.c_int32),
-                ("words", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("words", C.c_int32), ("learn_margin", C.c_float), ("reserved", C.c_int32 * 2)]
 
 
 class BatchStats(C.Structure):
@@ -111,7 +111,8 @@ _sig("fc_stcon_sched_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_in
      C.POINTER(Stats))
 _sig("fc_stcon_batch", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(BatchOpts),
      C.POINTER(BatchStats))
-_sig("fc_sched_trace", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.POINTER(TraceRow), C.c_int32, _i32p, _i32p)
+_sig("fc_sched_trace", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(TraceRow),
+     C.c_int32, _i32p, _i32p)
 _sig("fc_sched_train", C.c_int, C.c_void_p, C.POINTER(TraceRow), C.c_int32, C.c_int32, C.c_float, C.c_uint64,
      _f32p)
 _sig("fc_sched_set_params", C.c_int, C.c_void_p, _f32p, C.c_int32)
@@ -309,19 +310,19 @@ class Graph:
         return StResult(conn, dist, st.as_dict())
 
     def stcon_batch(self, s, t, policy: int = POLICY_HEURISTIC, vw: int = 0, tile: int = 0, use_graph: int = 1,
-                    words: int = 1) -> StResult:
+                    words: int = 1, learn_margin: float = -1.0) -> StResult:
         s = np.ascontiguousarray(s, np.int32)
         t = np.ascontiguousarray(t, np.int32)
         conn = np.empty(len(s), np.int32)
         dist = np.empty(len(s), np.int32)
-        o = BatchOpts(policy, vw, tile, use_graph, words)
+        o = BatchOpts(policy, vw, tile, use_graph, words, learn_margin)
         st = BatchStats()
         _check(_lib.fc_stcon_batch(self._h, _p32(s), _p32(t), len(s), _p32(conn), _p32(dist), C.byref(o),
                                    C.byref(st)))
         return StResult(conn, dist, st.as_dict())
 
     # ---- learned scheduler ----
-    def sched_trace(self, s, t, cap: int = 4096):
+    def sched_trace(self, s, t, cap: int = 4096, every: int = 1, policy: int = POLICY_HEURISTIC):
         """Trace the heuristic schedule over the queries. Returns (features
         [rows, 12], cost_us [rows, 6], heuristic action [rows], wave [rows],
         distances [n])."""
@@ -330,7 +331,8 @@ class Graph:
         rows = (TraceRow * max(cap, 1))()
         n = C.c_int32()
         dist = np.empty(len(s), np.int32)
-        _check(_lib.fc_sched_trace(self._h, _p32(s), _p32(t), len(s), rows, cap, C.byref(n), _p32(dist)))
+        _check(_lib.fc_sched_trace(self._h, _p32(s), _p32(t), len(s), policy, every, rows, cap, C.byref(n),
+                                   _p32(dist)))
         k = n.value
         arr = np.ctypeslib.as_array(rows)[:k] if k else None
         feats = np.zeros((k, SCHED_FEATURES), np.float32)

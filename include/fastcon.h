@@ -205,7 +205,9 @@ typedef struct fc_batch_opts {
   int32_t tile;          /* tile schedule (work items per wave-iteration); 0 = autotuned */
   int32_t use_graph;     /* 1: replay a captured hipGraph per wave; 0: plain launches */
   int32_t words;         /* 64-bit query words per vertex (1 = 64 queries per wave) */
-  int32_t reserved[3];
+  float   learn_margin;  /* learned policy: the network overrides the heuristic's action only
+                            when it predicts a log-cost lower by at least this; < 0 = 0.1 */
+  int32_t reserved[2];
 } fc_batch_opts;
 
 typedef struct fc_batch_stats {
@@ -248,7 +250,7 @@ fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t* t, int32_
 
 /* Per-step trace record logged by timing every alternative from one state:
  * cost_us[a] is the device time from taking action a at this step to the
- * end of the wave, the rest of the wave following the heuristic. */
+ * end of the wave, the rest of the wave following the traced schedule. */
 typedef struct fc_trace_row {
   float features[FC_SCHED_FEATURES];
   float cost_us[FC_SCHED_ACTIONS];   /* microseconds */
@@ -256,11 +258,13 @@ typedef struct fc_trace_row {
   int32_t wave;                      /* wave index within the traced batch */
 } fc_trace_row;
 
-/* Run the batched engine along the heuristic schedule, pausing at every
- * step decision to time all FC_SCHED_ACTIONS alternatives (device state is
- * snapshot and restored around each), and append up to cap rows. Returns the
- * number written in *n_rows; dist_out (nullable) receives the distances. */
-fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
+/* Run the batched engine along the heuristic (or learned, policy
+ * FC_POLICY_LEARNED, no fallback) schedule, pausing at every
+ * `every`-th step decision to time all FC_SCHED_ACTIONS alternatives (device
+ * state is copied and restored around each), until cap rows are written;
+ * the rest runs untraced. Returns the number written in *n_rows; dist_out
+ * (nullable) receives the distances. */
+fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy, int32_t every,
                          fc_trace_row* rows, int32_t cap, int32_t* n_rows, int32_t* dist_out);
 /* Train on the device (hand-written forward/backward/Adam kernels), full
  * batch, from a seeded initialisation. Targets are log(cost_us) minus the

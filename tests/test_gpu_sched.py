@@ -103,3 +103,28 @@ def test_trace_keeps_answers_and_times_every_action(fc, orc):
         # the traced batch still answers correctly afterwards, under every policy
         for pol in (0, 1):
             assert np.array_equal(g.stcon_batch(s, t, policy=pol).dist, exp)
+
+
+def test_trained_network_gains_over_plain_schedule(fc, orc):
+    """The shipped trained parameters on a graph and queries they were not
+    trained on: every answer equals the oracle's, and the learned schedule
+    beats the plain schedule (both frontiers top-down every step, the
+    reference's order of work)."""
+    hg = fc.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=77)
+    r = hg.sample_roots(1024, 78)
+    s, t = r[:512], r[512:]
+    exp = _oracle_dist(orc, hg, s[:48], t[:48])
+    with fc.Graph(hg) as g:
+        assert fc.load_default_sched_params(g)
+        ms = {}
+        for name, pol in (("plain", fc.POLICY_PLAIN), ("heuristic", fc.POLICY_HEURISTIC),
+                          ("learned", fc.POLICY_LEARNED)):
+            g.stcon_batch(s, t, policy=pol)
+            times = []
+            for _ in range(5):
+                res = g.stcon_batch(s, t, policy=pol)
+                assert np.array_equal(res.dist[:48], exp), name
+                times.append(res.stats["time_ms"])
+            ms[name] = float(np.median(times))
+        say(f"rmat20 512 queries, ms per batch: {ms}; learned gains {ms['plain'] / ms['learned']:.2f}x over plain")
+        assert ms["learned"] * 1.1 < ms["plain"]
