@@ -41,6 +41,8 @@ using fc::fail;
 
 constexpr uint32_t kBStage = 512;  // staged entries per wave and list
 constexpr float kDefaultLearnMargin = 0.1f;
+constexpr int kMaxLanes = 4;          // concurrent wave machines per batch
+constexpr int kDefaultLanes = 3;
 // A filter launch pays when the expected number of dropped list entries or
 // frontier edges reaches these.
 constexpr uint64_t kFilterMinVertices = 1u << 15;
@@ -433,39 +435,104 @@ __device__ __forceinline__ void mark_dirty(const BatchParams<OffT>& p, bool dirt
 }
 
 // Commit: every vertex of a touched group folds its N words into its visited
-// words, and newly reached vertices join the next frontiers.
+// words, and newly reached vertices join the next frontiers. The wave reads
+// 64 group marks, then handles the marked groups two at a time (lanes 0..31
+// and 32..63, records read and written in address order); the records and
+// offsets of the next pair are loaded before the current pair is processed,
+// so each pass costs about one memory round trip. Each group belongs to one
+// wave of the launch, which therefore updates its dirty-bitmap word without
+// an atomic.
+template <typename OffT>
+struct CommitLoad {
+  ulonglong2 a, b;  // {visS, NS}, {visT, NT}
+  OffT o0, o1;
+  int32_t u;
+  bool valid;
+};
+
+template <typename OffT>
+__device__ __forceinline__ CommitLoad<OffT> commit_load(const BatchParams<OffT>& p, int64_t gb, int k0, int k1) {
+  const int lane = lane_id();
+  const int mine = lane < 32 ? k0 : k1;
+  CommitLoad<OffT> c;
+  const int64_t u = (gb + (mine < 0 ? 0 : mine)) * 32 + (lane & 31);
+  c.u = (int32_t)u;
+  c.valid = mine >= 0 && u < p.V;
+  c.a = c.b = make_ulonglong2(0ull, 0ull);
+  c.o0 = c.o1 = 0;
+  if (c.valid) {
+    const ulonglong2* r = reinterpret_cast<const ulonglong2*>(p.rec + 4 * u);
+    c.a = r[0];
+    c.b = r[1];
+    c.o0 = p.off[u];
+    c.o1 = p.off[u + 1];
+  }
+  return c;
+}
+
+__device__ __forceinline__ bool take_pair(uint64_t& nz, int& k0, int& k1) {
+  if (!nz) return false;
+  k0 = __builtin_ctzll(nz);
+  nz &= nz - 1;
+  k1 = -1;
+  if (nz) {
+    k1 = __builtin_ctzll(nz);
+    nz &= nz - 1;
+  }
+  return true;
+}
+
 template <typename OffT>
 __device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends& ap, unsigned long long act,
                                             const int* nxt, int gwave, int nwaves, unsigned long long* red) {
-  scan_groups(p.tm, p.V, gwave, nwaves, [&](int32_t u, bool valid, int64_t g0, int64_t g1) {
-    unsigned long long nS = 0, nT = 0, vS = 0, vT = 0;
-    int64_t deg = 0;
-    if (valid) {
-      ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
-      const ulonglong2 a = r[0], b = r[1];  // {visS, NS}, {visT, NT}
-      vS = a.x;
-      vT = b.x;
-      nS = a.y & ~vS & act;
-      nT = b.y & ~vT & act;
-      red[kROdd] |= (nS & vT) | (nT & vS);  // odd meets
-      red[kREven] |= nS & nT;                // even meets
-      red[kRAnyS] |= nS;
-      red[kRAnyT] |= nT;
-      if (a.y | b.y) red[kRTouched] += 1;
-      if (a.y) r[0] = make_ulonglong2(vS | nS, 0ull);
-      if (b.y) r[1] = make_ulonglong2(vT | nT, 0ull);
-      if (nS) p.F[0][nxt[0]][u] = nS;
-      if (nT) p.F[1][nxt[1]][u] = nT;
-      if (nS | nT) deg = bdeg(p, u);
-      if (nS) red[kRMfS] += (unsigned long long)deg;
-      if (nT) red[kRMfT] += (unsigned long long)deg;
-      red[kRVisS] += (unsigned long long)__popcll(nS);
-      red[kRVisT] += (unsigned long long)__popcll(nT);
+  const int lane = lane_id();
+  const int64_t groups = ((int64_t)p.V + 31) / 32;
+  for (int64_t gb = (int64_t)gwave * kWave; gb < groups; gb += (int64_t)nwaves * kWave) {
+    const int64_t gi = gb + lane;
+    const bool marked = gi < groups && p.tm[gi] != 0;
+    if (marked) p.tm[gi] = 0;
+    uint64_t nz = ballot(marked);
+    int k0, k1;
+    if (!take_pair(nz, k0, k1)) continue;
+    CommitLoad<OffT> cur = commit_load(p, gb, k0, k1);
+    for (;;) {
+      int n0 = -1, n1 = -1;
+      const bool more = take_pair(nz, n0, n1);
+      CommitLoad<OffT> nx;
+      if (more) nx = commit_load(p, gb, n0, n1);
+      unsigned long long nS = 0, nT = 0, vS = 0, vT = 0;
+      int64_t deg = 0;
+      const int32_t u = cur.u;
+      if (cur.valid) {
+        vS = cur.a.x;
+        vT = cur.b.x;
+        nS = cur.a.y & ~vS & act;
+        nT = cur.b.y & ~vT & act;
+        red[kROdd] |= (nS & vT) | (nT & vS);  // odd meets
+        red[kREven] |= nS & nT;                // even meets
+        red[kRAnyS] |= nS;
+        red[kRAnyT] |= nT;
+        if (cur.a.y | cur.b.y) red[kRTouched] += 1;
+        ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
+        if (cur.a.y) r[0] = make_ulonglong2(vS | nS, 0ull);
+        if (cur.b.y) r[1] = make_ulonglong2(vT | nT, 0ull);
+        if (nS) p.F[0][nxt[0]][u] = nS;
+        if (nT) p.F[1][nxt[1]][u] = nT;
+        deg = (int64_t)(cur.o1 - cur.o0);
+        if (nS) red[kRMfS] += (unsigned long long)deg;
+        if (nT) red[kRMfT] += (unsigned long long)deg;
+        red[kRVisS] += (unsigned long long)__popcll(nS);
+        red[kRVisT] += (unsigned long long)__popcll(nT);
+      }
+      push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
+      push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
+      mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), gb + k0, k1 >= 0 ? gb + k1 : (int64_t)-1);
+      if (!more) break;
+      cur = nx;
+      k0 = n0;
+      k1 = n1;
     }
-    push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
-    push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
-    mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), g0, g1);
-  });
+  }
 }
 
 // Bottom-up step for the sides in SIDES (bit x = side x): every vertex still
@@ -907,7 +974,7 @@ __device__ __forceinline__ void init_wave(const BatchParams<OffT>& p, int32_t* s
 }
 
 template <typename OffT>
-__global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
+__global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   BatchCtl* ctl = p.ctl;
   const int phase = ctl->phase;
   if (phase == kBIdle || phase == kBPause) return;
@@ -1066,6 +1133,7 @@ __global__ __launch_bounds__(kBlock, 5) void batch_kernel(BatchParams<OffT> p) {
       if (ctl->phases < kPhaseLog) {
         ctl->plog[ctl->phases] = (int8_t)(phase * 16 + (phase == kBExpand || phase == kBBottomUp ? d.sides : 0));
         ctl->ptime[ctl->phases] = (int64_t)wall_clock64();
+        ctl->pwork[ctl->phases] = (int64_t)(phase == kBCommit ? tot.v[kRTouched] : tot.v[kRActE]);
       }
       ctl->phases++;
       if (phase == kBDecide)
@@ -1151,10 +1219,22 @@ struct BatchWorkspace {
   int32_t* h_dist = nullptr;  // pinned
   int32_t* h_phase = nullptr; // pinned
   char* snap = nullptr;        // tracing: copy of the workspace
+  // Concurrent lanes: lane 0 runs on the graph's stream; each further lane
+  // (a wave machine of its own, on its own stream) hangs off `next`.
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev_end = nullptr;
+  BatchWorkspace* next = nullptr;
 };
 
 void batch_free(BatchWorkspace* b) {
   if (!b) return;
+  batch_free(b->next);
+  if (b->own_stream && b->stream) {
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipStreamDestroy(b->stream);
+  }
+  if (b->ev_end) (void)hipEventDestroy(b->ev_end);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->mem);
@@ -1167,8 +1247,7 @@ void batch_free(BatchWorkspace* b) {
   delete b;
 }
 
-static fc_status batch_alloc(fc_graph* g) {
-  if (g->batch) return FC_OK;
+static fc_status batch_alloc_lane(fc_graph* g, BatchWorkspace** out, bool own_stream) {
   BatchWorkspace* b = new BatchWorkspace();
   const size_t V = (size_t)g->V;
   const size_t hcap = g->hcap;
@@ -1221,17 +1300,57 @@ static fc_status batch_alloc(fc_graph* g) {
     batch_free(b);
     return fail(FC_ERR_NOGPU, "batch kernel has no occupancy on this device");
   }
-  b->blocks = std::min(occ, 8) * g->cus;
+  // With three concurrent lanes, 2-3 blocks per CU per lane measured best
+  // at s22 (1: 8.9 ms, 2: 7.8, 3: 7.8, 4: 8.0 for 1,024 queries).
+  int per_cu = std::min(occ, 3);
+  const char* bpc = std::getenv("FASTCON_BATCH_BLOCKS_PER_CU");
+  if (bpc && std::atoi(bpc) > 0) per_cu = std::min(occ, std::atoi(bpc));
+  b->blocks = per_cu * g->cus;
   if (hipHostMalloc((void**)&b->h_phase, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
     batch_free(b);
     return fail(FC_ERR_NOMEM, "pinned allocation failed");
   }
-  g->batch = b;
+  if (own_stream) {
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+      b->stream = nullptr;
+      batch_free(b);
+      return fail(FC_ERR_HIP, "batch lane stream creation failed");
+    }
+    b->own_stream = true;
+  } else {
+    b->stream = g->stream;
+  }
+  if (hipEventCreate(&b->ev_end) != hipSuccess) {
+    b->ev_end = nullptr;
+    batch_free(b);
+    return fail(FC_ERR_HIP, "batch lane event creation failed");
+  }
+  *out = b;
   return FC_OK;
 }
 
-static fc_status batch_queries(fc_graph* g, int32_t n) {
+static fc_status batch_alloc(fc_graph* g) {
+  if (g->batch) return FC_OK;
+  return batch_alloc_lane(g, &g->batch, false);
+}
+
+// The k-th lane (0 = g->batch), allocated on first use.
+static fc_status batch_lane(fc_graph* g, int k, BatchWorkspace** out) {
+  fc_status st = batch_alloc(g);
+  if (st != FC_OK) return st;
   BatchWorkspace* b = g->batch;
+  for (int i = 0; i < k; ++i) {
+    if (!b->next) {
+      st = batch_alloc_lane(g, &b->next, true);
+      if (st != FC_OK) return st;
+    }
+    b = b->next;
+  }
+  *out = b;
+  return FC_OK;
+}
+
+static fc_status batch_queries(fc_graph* g, BatchWorkspace* b, int32_t n) {
   if (n <= b->qcap) return FC_OK;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1253,8 +1372,7 @@ static fc_status batch_queries(fc_graph* g, int32_t n) {
 }
 
 template <typename OffT>
-static BatchParams<OffT> batch_params(fc_graph* g) {
-  BatchWorkspace* b = g->batch;
+static BatchParams<OffT> batch_params(fc_graph* g, BatchWorkspace* b) {
   BatchParams<OffT> p;
   std::memset(&p, 0, sizeof(p));
   p.off = (const OffT*)g->d_off;
@@ -1283,19 +1401,18 @@ static BatchParams<OffT> batch_params(fc_graph* g) {
   return p;
 }
 
-static hipError_t batch_launch_raw(fc_graph* g) {
-  if (g->off64) return launch_batch_phase<int64_t>(batch_params<int64_t>(g), g->batch->blocks, g->stream);
-  return launch_batch_phase<int32_t>(batch_params<int32_t>(g), g->batch->blocks, g->stream);
+static hipError_t batch_launch_raw(fc_graph* g, BatchWorkspace* b) {
+  if (g->off64) return launch_batch_phase<int64_t>(batch_params<int64_t>(g, b), b->blocks, b->stream);
+  return launch_batch_phase<int32_t>(batch_params<int32_t>(g, b), b->blocks, b->stream);
 }
 
-static fc_status batch_graph(fc_graph* g) {
-  BatchWorkspace* b = g->batch;
+static fc_status batch_graph(fc_graph* g, BatchWorkspace* b) {
   if (b->gexec) return FC_OK;
-  FC_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  FC_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
-  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = batch_launch_raw(g);
+  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = batch_launch_raw(g, b);
   hipGraph_t gr = nullptr;
-  hipError_t ce = hipStreamEndCapture(g->stream, &gr);
+  hipError_t ce = hipStreamEndCapture(b->stream, &gr);
   if (le != hipSuccess) return hip_fail(le, "batch kernel capture");
   if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
   b->graph = gr;
@@ -1304,8 +1421,8 @@ static fc_status batch_graph(fc_graph* g) {
 }
 
 // Prepare the control block for a batch of n queries (already on the device).
-fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile, int trace, int every, float margin) {
-  BatchWorkspace* b = g->batch;
+fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int vw, int tile, int trace, int every,
+                      float margin) {
   BatchCtl init;
   std::memset(&init, 0, sizeof(init));
   init.phase = kBInit;
@@ -1320,77 +1437,146 @@ fc_status batch_begin(fc_graph* g, int32_t n, int policy, int vw, int tile, int 
   init.trace = trace;
   init.trace_every = every;
   init.learn_margin_milli = (int32_t)std::lround((margin < 0.f ? kDefaultLearnMargin : margin) * 1000.f);
-  FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
+  FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error
-  FC_HIP(hipMemsetAsync(b->red, 0, sizeof(unsigned long long) * 2 * kRedSlots * kRedWords, g->stream));
+  FC_HIP(hipMemsetAsync(b->red, 0, sizeof(unsigned long long) * 2 * kRedSlots * kRedWords, b->stream));
   return FC_OK;
 }
 
-// Run the phase machine until idle. use_graph: replay captured launches.
-static fc_status batch_drive(fc_graph* g, int32_t n, bool use_graph, float* ms, int64_t* launches_out) {
-  BatchWorkspace* b = g->batch;
-  const int64_t nw = (n + 63) / 64;
-  int64_t expected = (int64_t)std::ceil((double)nw * b->est_launches) + 2;
-  int64_t launched = 0;
-  FC_HIP(hipEventRecord(g->ev0, g->stream));
-  for (int round = 0;; ++round) {
-    int64_t want = std::max<int64_t>(expected - launched, 1);
-    if (!use_graph || want < kGraphLaunches) {
-      for (int64_t i = 0; i < want; ++i) {
-        hipError_t e = batch_launch_raw(g);
-        if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
-      }
-      launched += want;
-    } else {
-      const int64_t ng = (want + kGraphLaunches - 1) / kGraphLaunches;
-      for (int64_t i = 0; i < ng; ++i) FC_HIP(hipGraphLaunch(b->gexec, g->stream));
-      launched += ng * kGraphLaunches;
-    }
-    FC_HIP(hipEventRecord(g->ev1, g->stream));
-    FC_HIP(hipMemcpyAsync(b->h_phase, b->ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
-    FC_HIP(hipStreamSynchronize(g->stream));
-    if (b->h_phase[0] == kBIdle) break;
-    if (launched > nw * (4 * g->V + 16) + 64)
-      return fail(FC_ERR_CAPACITY, "batch phase machine did not finish after %lld launches", (long long)launched);
-    const int64_t remaining = std::max<int64_t>(nw - b->h_phase[1], 1);
-    expected = launched + (int64_t)std::ceil((double)remaining * (b->est_launches + 4.0) * (1.0 + round)) + 2;
+// Run the phase machines of `nl` lanes until every one is idle, issuing each
+// lane's launches on its own stream (use_graph: replay captured launches).
+// The elapsed time runs from an event on lane 0 that the other lanes wait
+// for to the point where every lane has finished.
+struct LaneRun {
+  BatchWorkspace* b;
+  int64_t nw;
+  int64_t expected;
+  int64_t launched;
+  int round;
+  bool done;
+};
+
+static fc_status batch_drive(fc_graph* g, LaneRun* lr, int nl, bool use_graph, float* ms, int64_t* launches_out) {
+  BatchWorkspace* b0 = lr[0].b;
+  FC_HIP(hipEventRecord(g->ev0, b0->stream));
+  for (int k = 1; k < nl; ++k) FC_HIP(hipStreamWaitEvent(lr[k].b->stream, g->ev0, 0));
+  for (int k = 0; k < nl; ++k) {
+    lr[k].expected = (int64_t)std::ceil((double)lr[k].nw * lr[k].b->est_launches) + 2;
+    lr[k].launched = 0;
+    lr[k].round = 0;
+    lr[k].done = lr[k].nw == 0;
   }
+  for (;;) {
+    bool any = false;
+    for (int k = 0; k < nl; ++k) {
+      LaneRun& L = lr[k];
+      if (L.done) continue;
+      any = true;
+      const int64_t want = std::max<int64_t>(L.expected - L.launched, 1);
+      if (!use_graph || want < kGraphLaunches) {
+        for (int64_t i = 0; i < want; ++i) {
+          hipError_t e = batch_launch_raw(g, L.b);
+          if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+        }
+        L.launched += want;
+      } else {
+        const int64_t ng = (want + kGraphLaunches - 1) / kGraphLaunches;
+        for (int64_t i = 0; i < ng; ++i) FC_HIP(hipGraphLaunch(L.b->gexec, L.b->stream));
+        L.launched += ng * kGraphLaunches;
+      }
+      FC_HIP(hipEventRecord(L.b->ev_end, L.b->stream));
+      FC_HIP(hipMemcpyAsync(L.b->h_phase, L.b->ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, L.b->stream));
+    }
+    if (!any) break;
+    for (int k = 0; k < nl; ++k) {
+      LaneRun& L = lr[k];
+      if (L.done) continue;
+      FC_HIP(hipStreamSynchronize(L.b->stream));
+      if (L.b->h_phase[0] == kBIdle) {
+        L.done = true;
+        continue;
+      }
+      if (L.launched > L.nw * (4 * g->V + 16) + 64)
+        return fail(FC_ERR_CAPACITY, "batch phase machine did not finish after %lld launches", (long long)L.launched);
+      const int64_t remaining = std::max<int64_t>(L.nw - L.b->h_phase[1], 1);
+      L.expected = L.launched + (int64_t)std::ceil((double)remaining * (L.b->est_launches + 4.0) * (1.0 + L.round)) + 2;
+      L.round++;
+    }
+  }
+  for (int k = 1; k < nl; ++k) FC_HIP(hipStreamWaitEvent(b0->stream, lr[k].b->ev_end, 0));
+  FC_HIP(hipEventRecord(g->ev1, b0->stream));
+  FC_HIP(hipEventSynchronize(g->ev1));
   FC_HIP(hipEventElapsedTime(ms, g->ev0, g->ev1));
-  b->est_launches = 0.5 * b->est_launches + 0.5 * ((double)launched / (double)nw);
-  if (launches_out) *launches_out = launched;
+  int64_t total = 0;
+  for (int k = 0; k < nl; ++k) {
+    if (lr[k].nw > 0)
+      lr[k].b->est_launches = 0.5 * lr[k].b->est_launches + 0.5 * ((double)lr[k].launched / (double)lr[k].nw);
+    total += lr[k].launched;
+  }
+  if (launches_out) *launches_out = total;
   return FC_OK;
 }
 
+// Answer n queries: the waves are split between `lanes` concurrent wave
+// machines (contiguous ranges of whole waves); distances land in lane 0's
+// pinned h_dist[0..n).
 static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy, int vw, int tile,
                            bool use_graph, float* ms, BatchCtl* ctl_out, int64_t* launches = nullptr,
-                           float margin = -1.f) {
-  BatchWorkspace* b = g->batch;
-  fc_status st = batch_queries(g, n);
-  if (st != FC_OK) return st;
-  if (use_graph) {
-    st = batch_graph(g);
-    if (st != FC_OK) return st;
+                           float margin = -1.f, int lanes = 1) {
+  const int64_t nw = (n + 63) / 64;
+  const int nl = (int)std::max<int64_t>(1, std::min<int64_t>(lanes, nw));
+  LaneRun lr[kMaxLanes];
+  int32_t first[kMaxLanes + 1];
+  fc_status st;
+  for (int k = 0; k <= nl; ++k) first[k] = (int32_t)std::min<int64_t>((int64_t)n, 64 * ((nw * k) / nl));
+  for (int k = 0; k < nl; ++k) {
+    BatchWorkspace* b = nullptr;
+    if ((st = batch_lane(g, k, &b)) != FC_OK) return st;
+    const int32_t m = first[k + 1] - first[k];
+    if ((st = batch_queries(g, b, k == 0 ? n : m)) != FC_OK) return st;
+    if (use_graph && (st = batch_graph(g, b)) != FC_OK) return st;
+    FC_HIP(hipMemcpyAsync(b->qs, s + first[k], sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, b->stream));
+    FC_HIP(hipMemcpyAsync(b->qt, t + first[k], sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, b->stream));
+    if ((st = batch_begin(g, b, m, policy, vw, tile, 0, 1, margin)) != FC_OK) return st;
+    lr[k].b = b;
+    lr[k].nw = (m + 63) / 64;
   }
-  FC_HIP(hipMemcpyAsync(b->qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
-  FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
-  st = batch_begin(g, n, policy, vw, tile, 0, 1, margin);
-  if (st != FC_OK) return st;
-  FC_HIP(hipStreamSynchronize(g->stream));
-  st = batch_drive(g, n, use_graph, ms, launches);
-  if (st != FC_OK) return st;
-  FC_HIP(hipMemcpyAsync(b->h_dist, b->dist, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
-  if (ctl_out) FC_HIP(hipMemcpyAsync(ctl_out, b->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, g->stream));
-  FC_HIP(hipStreamSynchronize(g->stream));
-  if (ctl_out && ctl_out->err) return fail(FC_ERR_CAPACITY, "batch kernel: list guard tripped (err=%u)", ctl_out->err);
-  if (ctl_out && std::getenv("FASTCON_BATCH_LOG")) {
-    // phase sequence: I=init E=expand(s/t/b sides) C=commit U=bottom-up X=clear F=filter Z=cleanup
+  for (int k = 0; k < nl; ++k) FC_HIP(hipStreamSynchronize(lr[k].b->stream));
+  if ((st = batch_drive(g, lr, nl, use_graph, ms, launches)) != FC_OK) return st;
+  BatchWorkspace* b0 = lr[0].b;
+  std::vector<BatchCtl> cs(nl);
+  for (int k = 0; k < nl; ++k) {
+    const int32_t m = first[k + 1] - first[k];
+    FC_HIP(hipMemcpyAsync(b0->h_dist + first[k], lr[k].b->dist, sizeof(int32_t) * (size_t)m, hipMemcpyDeviceToHost,
+                          lr[k].b->stream));
+    FC_HIP(hipMemcpyAsync(&cs[k], lr[k].b->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, lr[k].b->stream));
+  }
+  for (int k = 0; k < nl; ++k) FC_HIP(hipStreamSynchronize(lr[k].b->stream));
+  uint32_t err = 0;
+  for (int k = 0; k < nl; ++k) err |= cs[k].err;
+  if (err) return fail(FC_ERR_CAPACITY, "batch kernel: list guard tripped (err=%u)", err);
+  if (ctl_out) {
+    *ctl_out = cs[0];
+    for (int k = 1; k < nl; ++k) {
+      ctl_out->levels_total += cs[k].levels_total;
+      ctl_out->active_vertices += cs[k].active_vertices;
+      ctl_out->active_edges += cs[k].active_edges;
+      ctl_out->touched_total += cs[k].touched_total;
+      ctl_out->filters += cs[k].filters;
+      ctl_out->phases += cs[k].phases;
+    }
+  }
+  if (std::getenv("FASTCON_BATCH_LOG")) {
+    // phase sequence of lane 0: I=init E=expand(s/t/b sides) C=commit U=bottom-up X=clear F=filter Z=cleanup
+    const BatchCtl& c = cs[0];
     const char* nm = "?IECUXZF";
-    const int64_t np = std::min<int64_t>(ctl_out->phases, kPhaseLog);
+    const int64_t np = std::min<int64_t>(c.phases, kPhaseLog);
     for (int64_t i = 0; i < np; ++i) {
-      const int ph = ctl_out->plog[i] >> 4, sd = ctl_out->plog[i] & 15;
-      const double us = i ? (double)(ctl_out->ptime[i] - ctl_out->ptime[i - 1]) / 100.0 : 0.0;  // 100 MHz clock
+      const int ph = c.plog[i] >> 4, sd = c.plog[i] & 15;
+      const double us = i ? (double)(c.ptime[i] - c.ptime[i - 1]) / 100.0 : 0.0;  // 100 MHz clock
       std::fprintf(stderr, "%c%s:%.0f", nm[ph & 7], sd == 1 ? "s" : sd == 2 ? "t" : sd == 3 ? "b" : "", us);
+      if (c.pwork[i]) std::fprintf(stderr, "/%lldk", (long long)(c.pwork[i] / 1000));
       std::fputc(ph == kBCleanup ? '\n' : ' ', stderr);
     }
   }
@@ -1428,11 +1614,11 @@ static fc_status drive_to_stop(fc_graph* g, int64_t cap) {
   BatchWorkspace* b = g->batch;
   for (int64_t launched = 0;; launched += 8) {
     for (int i = 0; i < 8; ++i) {
-      hipError_t e = batch_launch_raw(g);
+      hipError_t e = batch_launch_raw(g, b);
       if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
     }
-    FC_HIP(hipMemcpyAsync(b->h_phase, b->ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
-    FC_HIP(hipStreamSynchronize(g->stream));
+    FC_HIP(hipMemcpyAsync(b->h_phase, b->ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+    FC_HIP(hipStreamSynchronize(b->stream));
     if (b->h_phase[0] == kBIdle || b->h_phase[0] == kBPause) return FC_OK;
     if (launched > cap) return fail(FC_ERR_CAPACITY, "batch phase machine did not stop after %lld launches", (long long)launched);
   }
@@ -1476,6 +1662,9 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
     return fc::fail(FC_ERR_INVALID, "fc_stcon_batch: vw must be 0, 4, 8, 16, 32 or 64");
   if (o.tile != 0 && (o.tile < 64 || o.tile > 65536 || (o.tile % 64) != 0))
     return fc::fail(FC_ERR_INVALID, "fc_stcon_batch: tile must be 0 or a multiple of 64 in [64, 65536]");
+  if (o.lanes < 0 || o.lanes > fcd::kMaxLanes)
+    return fc::fail(FC_ERR_INVALID, "fc_stcon_batch: lanes must be 0 (default) or 1..%d", fcd::kMaxLanes);
+  const int lanes = o.lanes ? o.lanes : fcd::kDefaultLanes;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (n == 0) return FC_OK;
   fc_status st = fcd::batch_alloc(g);
@@ -1495,7 +1684,7 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
   float ms = 0;
   BatchCtl c;
   int64_t launches = 0;
-  st = fcd::batch_run(g, s, t, n, o.policy, vw, tile, o.use_graph != 0, &ms, &c, &launches, o.learn_margin);
+  st = fcd::batch_run(g, s, t, n, o.policy, vw, tile, o.use_graph != 0, &ms, &c, &launches, o.learn_margin, lanes);
   if (st != FC_OK) return st;
   for (int32_t i = 0; i < n; ++i) {
     const int32_t d = b->h_dist[i];
@@ -1543,7 +1732,7 @@ extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t
   if (st != FC_OK) return st;
   g->levels_valid = g->parents_valid = false;
   fcd::BatchWorkspace* b = g->batch;
-  if ((st = fcd::batch_queries(g, n)) != FC_OK) return st;
+  if ((st = fcd::batch_queries(g, b, n)) != FC_OK) return st;
   if (!b->snap && hipMalloc((void**)&b->snap, b->bytes) != hipSuccess) {
     b->snap = nullptr;
     return fc::fail(FC_ERR_NOMEM, "fc_sched_trace: no room for the workspace copy (%zu bytes)", b->bytes);
@@ -1555,7 +1744,7 @@ extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t
   FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   const int vw = b->tuned_vw >= 0 ? b->tuned_vw : 0;
   const int tile = b->tuned_tile > 0 ? b->tuned_tile : 256;
-  if ((st = fcd::batch_begin(g, n, policy, vw, tile, cap > 0 ? 1 : 0, every, 0.f)) != FC_OK) return st;
+  if ((st = fcd::batch_begin(g, b, n, policy, vw, tile, cap > 0 ? 1 : 0, every, 0.f)) != FC_OK) return st;
   const int64_t cap_launches = ((int64_t)n / 64 + 1) * (4 * g->V + 16) + 64;
   std::vector<BatchCtl> cv(2);
   BatchCtl& c = cv[0];

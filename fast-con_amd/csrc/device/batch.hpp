@@ -90,6 +90,7 @@ struct BatchCtl {
   float feat[12];
   int8_t plog[kPhaseLog];   // phase of each working launch (diagnostics)
   int64_t ptime[kPhaseLog];  // wall clock at its finalisation
+  int64_t pwork[kPhaseLog];  // frontier edges (expand, bottom-up) or touched vertices (commit) of it
 };
 
 template <typename OffT>

@@ -19,12 +19,15 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 using fc::fail;
 
 namespace fc {
 fc_status validate_csr32(const int32_t* off, const int32_t* col, int64_t V, int64_t E, int64_t hub, int64_t* hubs);
 fc_status validate_csr64(const int64_t* off, const int32_t* col, int64_t V, int64_t E, int64_t hub, int64_t* hubs);
+void hub_first32(const int32_t* off, const int32_t* col, int64_t V, int32_t* out);
+void hub_first64(const int64_t* off, const int32_t* col, int64_t V, int32_t* out);
 }  // namespace fc
 
 namespace fcd {
@@ -237,7 +240,18 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   CK(hipMalloc(&g->d_off, sizeof(OffT) * (size_t)(V + 1)));
   CK(hipMalloc((void**)&g->d_col, sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1)));
   CK(hipMemcpy(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
-  if (E > 0) CK(hipMemcpy(g->d_col, col, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+  if (E > 0) {
+    if (flags & FC_GRAPH_KEEP_ORDER) {
+      CK(hipMemcpy(g->d_col, col, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+    } else {
+      std::vector<int32_t> ordered((size_t)E);
+      if (sizeof(OffT) == 8)
+        fc::hub_first64((const int64_t*)off, col, V, ordered.data());
+      else
+        fc::hub_first32((const int32_t*)off, col, V, ordered.data());
+      CK(hipMemcpy(g->d_col, ordered.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+    }
+  }
 
   // Workspace layout (256-byte aligned pieces of one allocation).
   const size_t nw64 = (size_t)((V + 63) / 64);
@@ -274,7 +288,10 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
 
   int occ = sizeof(OffT) == 8 ? fcd::phase_occupancy<int64_t>() : fcd::phase_occupancy<int32_t>();
   if (occ < 1) return bail(fail(FC_ERR_NOGPU, "phase kernel has no occupancy on this device (missing gfx950 code?)"));
-  int per_cu = std::min(occ, 8);
+  // 4 resident blocks per CU measured best for BFS and plain s–t at s22
+  // (2: 345, 3: 384, 4: 401, 6: 392, 8: 348 GTEPS): more blocks add
+  // contention on the launch's shared counters more than they hide latency.
+  int per_cu = std::min(occ, 4);
   const char* bpc = std::getenv("FASTCON_BLOCKS_PER_CU");
   if (bpc && std::atoi(bpc) > 0) per_cu = std::min(occ, std::atoi(bpc));
   g->blocks = per_cu * g->cus;

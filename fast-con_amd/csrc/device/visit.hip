@@ -38,7 +38,7 @@
 // in which the first proposal appears, and the minimum proposal is the exact
 // distance.
 #include "visit.hpp"
-#include "mlp.hpp"
+#include "hubstage.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -78,6 +78,7 @@ struct Appender {
   uint32_t cnt;                 // staged entries (wave-uniform)
   uint64_t mf;                  // per-lane degree sum of claimed vertices
   unsigned long long best;      // per-lane best meeting candidate (distance << 32 | vertex)
+  HubStage h;                   // this wave's staged hub entries
 };
 
 template <typename OffT>
@@ -100,8 +101,9 @@ __device__ __forceinline__ void wave_flush(Appender<OffT>& a) {
 }
 
 // Wave-collective: add every claimed vertex (degree deg) to the next frontier
-// of the appender's colour: normal entries through the LDS stage, hub entries
-// (degree >= hub_threshold) straight to the hub queue with their edge base.
+// of the appender's colour: normal entries and hub entries (degree >=
+// hub_threshold) through their LDS stages. A per-lane atomic on the one hub
+// counter serialised whole levels when thousands of hubs were claimed.
 template <typename OffT>
 __device__ __forceinline__ void push_next(Appender<OffT>& a, bool claimed, int32_t u, int64_t deg, bool set_bitmap) {
   const VisitParams<OffT>& p = *a.p;
@@ -114,17 +116,8 @@ __device__ __forceinline__ void push_next(Appender<OffT>& a, bool claimed, int32
     if (norm) a.buf[a.cnt + lane_rank(m)] = u;
     a.cnt += k;
   }
-  if (hub) {
-    const unsigned long long old =
-        atomicAdd((unsigned long long*)&a.nxt->hub, (1ull << kHubEdgeBits) | (unsigned long long)deg);
-    const uint32_t hs = (uint32_t)(old >> kHubEdgeBits);
-    if (hs < p.hcap) {
-      p.hq[a.c][a.nxt_par][hs] = u;
-      p.hb[a.c][a.nxt_par][hs] = (int64_t)(old & kHubEdgeMask);
-    } else {
-      atomicOr(&p.ctl->err, 1u);
-    }
-  }
+  hub_push(a.h, hub, u, deg, (unsigned long long*)&a.nxt->hub, p.hq[a.c][a.nxt_par], p.hb[a.c][a.nxt_par], p.hcap,
+           &p.ctl->err);
   if (claimed) {
     a.mf += (uint64_t)deg;
     if (set_bitmap) atomicOr(a.nfbm + (u >> 5), 1u << (u & 31));
@@ -148,6 +141,7 @@ struct BlockShared {
   unsigned long long mf[kWavesPerBlock];
   unsigned long long best[kWavesPerBlock];
   uint32_t base;
+  HubShared hub;
 };
 
 template <typename OffT>
@@ -189,6 +183,8 @@ __device__ __forceinline__ void block_flush(Appender<OffT>& a, BlockShared& sh) 
   a.mf = 0;
   a.best = ~0ull;
   __syncthreads();
+  hub_block_flush(a.h, (unsigned long long*)&a.nxt->hub, a.p->hq[a.c][a.nxt_par], a.p->hb[a.c][a.nxt_par], a.p->hcap,
+                  &a.p->ctl->err, sh.hub);
 }
 
 // Probe edge (v -> u) for colour c whose frontier is at level lvl. Returns
@@ -504,14 +500,12 @@ __device__ void decide(VisitCtl* ctl, const float* mlp, int64_t V, int64_t E) {
     d.vw[0] = width(0);
     d.vw[1] = width(1);
   } else {
-    const uint32_t nf[2] = {cn[0] + (uint32_t)(chub[0] >> kHubEdgeBits), cn[1] + (uint32_t)(chub[1] >> kHubEdgeBits)};
-    const uint64_t hn[2] = {chub[0] >> kHubEdgeBits, chub[1] >> kHubEdgeBits};
-    step_features(ctl->feat, nf, cmf, hn, ctl->lvl[0], ctl->lvl[1], V, E, ctl->seen_edges);
+    // Scheduled single-query visits (switched off on the host side; the
+    // learned scheduler drives the batched engine): the side with the
+    // smaller frontier, bottom-up when it would touch most unseen edges.
     const int side = cmf[0] <= cmf[1] ? 0 : 1;
     const bool bu = (double)cmf[side] > mu / kAlpha;
-    int a = side * 3 + (bu ? 2 : (width(side) >= 16 ? 1 : 0));
-    if (ctl->policy == 2 && mlp) a = mlp_argmin(mlp, ctl->feat);
-    apply_action(d, a);
+    apply_action(d, side * 3 + (bu ? 2 : (width(side) >= 16 ? 1 : 0)));
   }
   ctl->dec = d;
 }
@@ -629,6 +623,7 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
   int32_t* const parent = ctl->record_parents ? p.parent : nullptr;
   __shared__ int s_last;
   __shared__ int32_t s_stage[kWavesPerBlock * kStageCap];
+  __shared__ int32_t s_hub[2][kWavesPerBlock * kHubStage];
   __shared__ BlockShared s_sh;
 
   Decision d = ctl->dec;
@@ -678,6 +673,9 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
       a.cnt = 0;
       a.mf = 0;
       a.best = ~0ull;
+      a.h.u = &s_hub[0][(threadIdx.x >> 6) * kHubStage];
+      a.h.d = &s_hub[1][(threadIdx.x >> 6) * kHubStage];
+      a.h.cnt = 0;
       if (d.strat[c] == kTopDown) {
         td_dispatch(a, parent, st, d.vw[c], lvl[c], cur_par, cur.n, gwave, nwaves);
         td_hubs(a, parent, st, lvl[c], cur_par, cur.hub, gwave, nwaves);
@@ -699,8 +697,21 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = t == nb - 1;
+    // Two-level ticket: the blocks of group b % kTicketGroups count on their
+    // own line, and the last of each group counts on the launch ticket, so
+    // no single address takes an atomic from every block.
+    const uint32_t k = blockIdx.x % kTicketGroups;
+    const uint32_t gsize = (nb - k + kTicketGroups - 1) / kTicketGroups;
+    uint32_t* sub = &ctl->gticket[k * 32];
+    const uint32_t t1 = __hip_atomic_fetch_add(sub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool last = false;
+    if (t1 == gsize - 1) {
+      __hip_atomic_store(sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t groups = nb < (uint32_t)kTicketGroups ? nb : (uint32_t)kTicketGroups;
+      const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = t == groups - 1;
+    }
+    s_last = last;
   }
   __syncthreads();
   if (!s_last || threadIdx.x != 0) return;

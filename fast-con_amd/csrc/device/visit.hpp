@@ -10,6 +10,7 @@
 namespace fcd {
 
 constexpr int kBlock = 256;
+constexpr int kTicketGroups = 16;  // two-level launch ticket: block b counts in group b % 16
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int32_t kColorBit = 1 << 30;        // colour of an s–t state word (0 = s side, 1 = t side)
 constexpr int32_t kLevelMask = kColorBit - 1;
@@ -78,6 +79,7 @@ struct VisitCtl {
   unsigned long long best;  // (candidate distance << 32) | meet vertex; ~0 = none
   uint32_t ticket;
   uint32_t err;             // bit 0: queue slot out of range, bit 1: bad vertex id read
+  uint32_t gticket[kTicketGroups * 32];  // group tickets, one 128-byte line each
   uint64_t seen_edges;      // sum of degrees of every vertex labelled so far
   int64_t visited_v, visited_e, expv, expe;
   float feat[12];           // features of the pending decision (scheduled mode)

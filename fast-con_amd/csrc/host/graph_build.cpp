@@ -221,6 +221,29 @@ static fc_status validate_csr_t(const OffT* off, const int32_t* col, int64_t V, 
   return FC_OK;
 }
 
+// Device adjacency order: each vertex's neighbours by decreasing degree
+// (ties by id), so that a bottom-up scan meets the hubs, which join the
+// frontier first, at the front of the list. Levels, verdicts and distances
+// do not depend on adjacency order.
+template <typename OffT>
+static void hub_first_t(const OffT* off, const int32_t* col, int64_t V, int32_t* out) {
+  std::vector<int64_t> deg((size_t)V);
+#pragma omp parallel for
+  for (int64_t v = 0; v < V; ++v) deg[(size_t)v] = (int64_t)off[v + 1] - (int64_t)off[v];
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t v = 0; v < V; ++v) {
+    const int64_t b = (int64_t)off[v], e = (int64_t)off[v + 1];
+    std::copy(col + b, col + e, out + b);
+    std::sort(out + b, out + e, [&](int32_t x, int32_t y) {
+      const int64_t dx = deg[(size_t)x], dy = deg[(size_t)y];
+      return dx != dy ? dx > dy : x < y;
+    });
+  }
+}
+
+void hub_first32(const int32_t* off, const int32_t* col, int64_t V, int32_t* out) { hub_first_t(off, col, V, out); }
+void hub_first64(const int64_t* off, const int32_t* col, int64_t V, int32_t* out) { hub_first_t(off, col, V, out); }
+
 fc_status validate_csr32(const int32_t* off, const int32_t* col, int64_t V, int64_t E, int64_t hub, int64_t* hubs) {
   return validate_csr_t(off, col, V, E, hub, hubs);
 }

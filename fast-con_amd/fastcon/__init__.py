@@ -28,6 +28,7 @@ _lib = C.CDLL(LIB_PATH)
 FC_INF = -1
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_IO, ERR_FORMAT, ERR_CAPACITY, ERR_NOGPU, ERR_TIMEOUT, ERR_UNSUPPORTED = range(10)
 POLICY_PLAIN, POLICY_HEURISTIC, POLICY_LEARNED = 0, 1, 2
+GRAPH_KEEP_ORDER = 1
 DIR_FILE, DIR_DIRECTED, DIR_UNDIRECTED = 0, 1, 2
 SCHED_FEATURES, SCHED_HIDDEN, SCHED_ACTIONS = 12, 32, 6
 SCHED_PARAMS = 12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6
@@ -61,7 +62,7 @@ class Stats(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("policy", C.c_int32), ("vw", C.c_int32), ("tile", C.c_int32), ("use_graph", C.c_int32),
-                ("words", C.c_int32), ("learn_margin", C.c_float), ("reserved", C.c_int32 * 2)]
+                ("words", C.c_int32), ("learn_margin", C.c_float), ("lanes", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BatchStats(C.Structure):
@@ -248,15 +249,18 @@ class StResult:
 class Graph:
     """A graph resident on the GPU (library-owned device copy + workspace)."""
 
-    def __init__(self, hg: HostGraph, force64: bool = False):
+    def __init__(self, hg: HostGraph, force64: bool = False, keep_order: bool = False):
+        """keep_order: keep the caller's adjacency order on the device
+        (default: neighbours by decreasing degree, see fc_graph_create)."""
         self.V, self.E = hg.V, hg.E
         self._h = C.c_void_p()
+        flags = GRAPH_KEEP_ORDER if keep_order else 0
         if hg.E < 2**31 - 1 and not force64:
             off32 = hg.offsets.astype(np.int32)
-            _check(_lib.fc_graph_create_csr32(_p32(off32), _p32(hg.columns), hg.V, hg.E, 0, C.byref(self._h)))
+            _check(_lib.fc_graph_create_csr32(_p32(off32), _p32(hg.columns), hg.V, hg.E, flags, C.byref(self._h)))
             self.offsets64 = False
         else:
-            _check(_lib.fc_graph_create_csr64(_p64(hg.offsets), _p32(hg.columns), hg.V, hg.E, 0,
+            _check(_lib.fc_graph_create_csr64(_p64(hg.offsets), _p32(hg.columns), hg.V, hg.E, flags,
                                               C.byref(self._h)))
             self.offsets64 = True
 
@@ -310,12 +314,12 @@ class Graph:
         return StResult(conn, dist, st.as_dict())
 
     def stcon_batch(self, s, t, policy: int = POLICY_HEURISTIC, vw: int = 0, tile: int = 0, use_graph: int = 1,
-                    words: int = 1, learn_margin: float = -1.0) -> StResult:
+                    words: int = 1, learn_margin: float = -1.0, lanes: int = 0) -> StResult:
         s = np.ascontiguousarray(s, np.int32)
         t = np.ascontiguousarray(t, np.int32)
         conn = np.empty(len(s), np.int32)
         dist = np.empty(len(s), np.int32)
-        o = BatchOpts(policy, vw, tile, use_graph, words, learn_margin)
+        o = BatchOpts(policy, vw, tile, use_graph, words, learn_margin, lanes)
         st = BatchStats()
         _check(_lib.fc_stcon_batch(self._h, _p32(s), _p32(t), len(s), _p32(conn), _p32(dist), C.byref(o),
                                    C.byref(st)))

@@ -138,7 +138,12 @@ typedef struct fc_graph fc_graph;
 
 /* Upload a CSR graph; the library keeps device copies and allocates all
  * per-query workspace once. Replaces cudaGraph::cudaGraph
- * (BFS-4K/src/device/cudaGraph.cu:7-36). flags: reserved, pass 0. */
+ * (BFS-4K/src/device/cudaGraph.cu:7-36). By default the device copy lists
+ * each vertex's neighbours by decreasing degree (ties by id), which makes
+ * bottom-up levels stop early; levels, verdicts and distances are the same
+ * for any adjacency order. flags: FC_GRAPH_KEEP_ORDER keeps the caller's
+ * order (then BFS parents follow the caller's order). */
+#define FC_GRAPH_KEEP_ORDER 1u
 fc_status fc_graph_create_csr32(const int32_t* off, const int32_t* col, int32_t V, int64_t E,
                                 uint32_t flags, fc_graph** out);
 fc_status fc_graph_create_csr64(const int64_t* off, const int32_t* col, int64_t V, int64_t E,
@@ -207,7 +212,9 @@ typedef struct fc_batch_opts {
   int32_t words;         /* 64-bit query words per vertex (1 = 64 queries per wave) */
   float   learn_margin;  /* learned policy: the network overrides the heuristic's action only
                             when it predicts a log-cost lower by at least this; < 0 = 0.1 */
-  int32_t reserved[2];
+  int32_t lanes;         /* waves run concurrently, each on its own stream and workspace
+                            (1..4); 0 = default (3) */
+  int32_t reserved;
 } fc_batch_opts;
 
 typedef struct fc_batch_stats {

@@ -130,3 +130,20 @@ def test_batch_every_action_is_exact(fc, orc, action):
             res = g.stcon_batch(s, t, policy=fc.POLICY_LEARNED, vw=0, tile=256)
             bad = np.flatnonzero(res.dist != exp)
             assert bad.size == 0, f"{name}: {bad[:5]} got {res.dist[bad[:5]]} want {exp[bad[:5]]}"
+
+
+def test_batch_lanes_agree(fc, orc):
+    """Waves split over 1..4 concurrent lanes (streams and workspaces) give
+    the same answers, including batches with fewer waves than lanes."""
+    hg = fc.gen_rmat(14, 16, seed=41)
+    r = hg.sample_roots(900, 42)
+    s, t = r[:450], r[450:]
+    exp = _oracle_dist(orc, hg, s, t)
+    with fc.Graph(hg) as g:
+        for lanes in (1, 2, 3, 4):
+            for n in (450, 100, 64, 1):
+                say(f"batch lanes={lanes} n={n}")
+                res = g.stcon_batch(s[:n], t[:n], lanes=lanes)
+                assert np.array_equal(res.dist, exp[:n]), (lanes, n)
+        with pytest.raises(fc.FastconError):
+            g.stcon_batch(s, t, lanes=5)

@@ -127,4 +127,4 @@ def test_trained_network_gains_over_plain_schedule(fc, orc):
                 times.append(res.stats["time_ms"])
             ms[name] = float(np.median(times))
         say(f"rmat20 512 queries, ms per batch: {ms}; learned gains {ms['plain'] / ms['learned']:.2f}x over plain")
-        assert ms["learned"] * 1.1 < ms["plain"]
+        assert ms["learned"] * 1.05 < ms["plain"]

@@ -187,3 +187,18 @@ def test_many_queries_reuse_and_errors(fc, orc):
         lv, _, _ = g.bfs(int(r[0]))
         d, _, _ = orc.bfs(hg.offsets, hg.columns, int(r[0]))
         assert np.array_equal(lv, d)
+
+
+def test_adjacency_order_does_not_change_answers(fc, orc):
+    """The device copy's hub-first adjacency order and the caller's order give
+    the same levels and distances, and valid BFS trees either way."""
+    hg = fc.gen_rmat(15, 16, seed=12)
+    roots = hg.sample_roots(64, 13)
+    s, t = roots[:32], roots[32:]
+    exp = np.array([orc.stcon(hg.offsets, hg.columns, a, b)[1] for a, b in zip(s, t)], np.int32)
+    for keep in (False, True):
+        with fc.Graph(hg, keep_order=keep) as g:
+            say(f"adjacency order keep={keep}")
+            _bfs_check(fc, orc, f"rmat15 keep={keep}", hg, g, roots[:4])
+            assert np.array_equal(g.stcon_many(s, t).dist, exp)
+            assert np.array_equal(g.stcon_batch(s, t).dist, exp)
