@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--policy", choices=["auto", "plain", "heuristic", "learned"], default="auto")
     ap.add_argument("--vw", type=int, default=0, help="virtual-warp width (0: autotuned)")
     ap.add_argument("--tile", type=int, default=0, help="hub chunk in edges (0: autotuned)")
+    ap.add_argument("--lanes", type=int, default=0, help="concurrent wave lanes (0: the library default, 3)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -124,7 +125,8 @@ def main():
 
     # ---- headline: batched s–t, K timed steps over the pairs ----
     def step():
-        return g.stcon_batch(s, t, policy=policy, vw=args.vw, tile=args.tile, use_graph=0 if args.no_graph else 1)
+        return g.stcon_batch(s, t, policy=policy, vw=args.vw, tile=args.tile, use_graph=0 if args.no_graph else 1,
+                             lanes=args.lanes)
 
     ref = step()  # first warm-up step; runs the width/tile autotuner unless both are given
     for _ in range(args.warmup - 1):
@@ -145,7 +147,9 @@ def main():
 
     # Roofline of the dominant kernel (batch_kernel, every phase of the wave
     # loop is one launch of it): algorithmic bytes per launch over the mean
-    # launch duration, both from the same timed region.
+    # launch interval (device time of the timed region / launches). With
+    # concurrent lanes, launches of different lanes overlap, so this interval
+    # is shorter than one kernel's own duration.
     launches = int(stats["launches"])
     bytes_run = batch_bytes(stats, V, g.offsets64)
     avg_launch_us = ms_step * 1e3 / launches
@@ -166,7 +170,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "fcd::batch_kernel<int>", "bytes_per_launch": int(bytes_run / launches),
                 "avg_launch_us": round(avg_launch_us, 2), "launches_per_step": launches,
-                "working_launches_per_step": int(stats["phases"])}
+                "working_launches_per_step": int(stats["phases"]), "lanes": args.lanes or 3}
 
     # ---- plain one-query-per-launch path on the same pairs ----
     plain = None
