@@ -1,0 +1,168 @@
+// bfs4K: the BFS-4K command line on fastcon.
+//
+//   bfs4K <graph_path> [ -D | -U ] [ -n <number_of_tests> ]
+//
+// Arguments, errors and the summary lines follow BFS-4K/bfs4K.cpp:58-81 and
+// BFS-4K/src/device/BFS_WorkEfficient.cu:6-66. Differences, each because the
+// reference's behaviour is not reproducible or not defined by its output:
+//   * sources are drawn from a seeded generator (environment FASTCON_SEED,
+//     default 1) instead of the time-seeded Mersenne twister; as in the
+//     reference, one test uses source 0 and sources reaching fewer than
+//     10,000 edges are redrawn on graphs with at least 10,000 vertices;
+//   * visited edges are counted per source (the reference reuses the first
+//     source's count for every later source);
+//   * "Avg. Enqueued Nodes" counts the engine's frontier entries, which hold
+//     no duplicates, so the ratio is 1.00x.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "fastcon.h"
+
+namespace {
+
+const char* kSyntax =
+    "Syntax Error:\n\n bfs4K <graph_path> [ <graph_direction> ] [ -n <number_of_tests>]\n\n"
+    "        <graph_direction>:\n"
+    "                        -D      force directed graph\n"
+    "                        -U      force undirected graph";
+
+[[noreturn]] void die(const std::string& msg) {
+  std::cerr << msg << std::endl;
+  std::exit(EXIT_FAILURE);
+}
+
+void check(fc_status st, const char* what) {
+  if (st != FC_OK) die(std::string(what) + ": " + fc_last_error());
+}
+
+// Thousands separators, as the reference's header line prints them.
+std::string grouped(long long v) {
+  std::string s = std::to_string(v < 0 ? -v : v), out;
+  for (size_t i = 0; i < s.size(); ++i) {
+    out.push_back(s[i]);
+    const size_t rest = s.size() - 1 - i;
+    if (rest && rest % 3 == 0) out.push_back(',');
+  }
+  return v < 0 ? "-" + out : out;
+}
+
+uint64_t splitmix(uint64_t& st) {
+  uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  if (argc < 2) die(kSyntax);
+  fc_direction dir = FC_DIR_FILE;
+  int nof_tests = 1;
+  for (int i = 2; i < argc; ++i) {
+    const std::string a = argv[i];
+    if (a == "-D")
+      dir = FC_DIR_DIRECTED;
+    else if (a == "-U")
+      dir = FC_DIR_UNDIRECTED;
+    else if (i + 1 < argc && a == "-n" && std::string(argv[i + 1]).find_first_not_of("0123456789") == std::string::npos &&
+             argv[i + 1][0] != '\0')
+      nof_tests = std::atoi(argv[++i]);
+    else
+      die(kSyntax);
+  }
+  if (nof_tests < 1) die(kSyntax);
+
+  const std::string path = argv[1];
+  struct stat sb;
+  if (stat(path.c_str(), &sb) != 0 || !S_ISREG(sb.st_mode)) die("Error. File not found or not regular: " + path);
+  std::cout << "\nRead Header:\t" << path.substr(path.find_last_of('/') + 1) << "\tSize: "
+            << grouped((long long)sb.st_size / (1024 * 1024)) << " MB\n";
+  std::cout << "Reading Graph File..." << std::flush;
+  fc_host_graph hg;
+  check(fc_read_graph(path.c_str(), dir, &hg), "read graph");
+  std::cout << "\tComplete!\n" << std::flush;
+  const long long V = hg.V, E = hg.E;
+  std::string gdir = hg.directed ? "GraphType: Directed " : "GraphType: Undirected ";
+  gdir += dir != FC_DIR_FILE ? "(User Def)" : "(File Def)";
+  std::cout << "\n\tNodes: " << grouped(V) << "\tEdges: " << grouped(E) << '\t' << gdir << "\tDegree AVG: " << std::fixed
+            << std::setprecision(1) << (V ? (float)E / (float)V : 0.f) << std::endl
+            << std::endl;
+
+  // degree analysis (BFS-4K/src/host/graph.cpp:140-150)
+  {
+    const float avg = V ? (float)E / (float)V : 0.f;
+    double var = 0;
+    long long mn = V ? (long long)1 << 62 : 0, mx = 0;
+    for (long long v = 0; v < V; ++v) {
+      const long long d = hg.offsets[v + 1] - hg.offsets[v];
+      var += ((double)d - avg) * ((double)d - avg);
+      mn = std::min(mn, d);
+      mx = std::max(mx, d);
+    }
+    const double sd = V ? std::sqrt(var / (double)V) : 0.0;
+    std::cout << std::setprecision(1) << "          Avg:  " << avg << std::endl
+              << "     Std. Dev:  " << sd << std::endl
+              << "          Min:  " << mn << "\t\t" << std::endl
+              << "          Max:  " << mx << "\t\t" << std::endl
+              << std::endl;
+  }
+  if (V == 0) die("Error. The graph has no vertices");
+
+  fc_graph* g = nullptr;
+  if (E < INT32_MAX) {
+    std::vector<int32_t> off32((size_t)V + 1);
+    for (long long v = 0; v <= V; ++v) off32[(size_t)v] = (int32_t)hg.offsets[v];
+    check(fc_graph_create_csr32(off32.data(), hg.columns, (int32_t)V, E, 0, &g), "upload graph");
+  } else {
+    check(fc_graph_create_csr64(hg.offsets, hg.columns, V, E, 0, &g), "upload graph");
+  }
+  std::cout << "Requested Device Memory: " << ((V + 1) * 4 + E * 4 + V * 4) / (1024 * 1024) << " MB\n\n";
+
+  const char* sv = std::getenv("FASTCON_SEED");
+  uint64_t rng = sv ? std::strtoull(sv, nullptr, 10) : 1;
+  long long totalVisitedEdges = 0, totalVisitedNodes = 0, diameter = 0, totalFrontierNodes = 0;
+  double totalTime = 0;
+  for (int i = 0; i < nof_tests; ++i) {
+    int32_t source = 0;
+    fc_stats st;
+    for (int attempt = 0;; ++attempt) {
+      source = nof_tests == 1 ? 0 : (int32_t)((double)(splitmix(rng) >> 11) * (1.0 / 9007199254740992.0) * (double)V);
+      check(fc_bfs(g, source, nullptr, nullptr, &st), "bfs");
+      if (nof_tests == 1 || V < 10000 || st.visited_edges >= 10000 || attempt > 1000) break;
+    }
+    totalVisitedNodes += st.visited_vertices;
+    totalVisitedEdges += st.visited_edges;
+    totalFrontierNodes += st.expanded_vertices;
+    diameter += st.levels > 0 ? st.levels - 1 : 0;
+    totalTime += st.time_ms;
+    if (nof_tests > 1) {
+      std::cout.unsetf(std::ios::floatfield);
+      std::cout << std::setprecision(6) << "iter: " << std::left << std::setw(10) << i << "time: " << std::setw(10)
+                << (float)st.time_ms << "Edges: " << std::setw(10) << st.visited_edges << "source: " << source
+                << std::endl;
+      std::cout << std::right;
+    }
+  }
+  std::cout << std::setprecision(1) << std::fixed << std::endl
+            << "\t    Number of TESTS:  " << nof_tests << std::endl
+            << "\t          Avg. Time:  " << totalTime / nof_tests << " ms" << std::endl
+            << "\t         Avg. MTEPS:  " << (double)totalVisitedEdges / (totalTime * 1000) << std::endl
+            << "\t      Avg. Diameter:  " << diameter / nof_tests << std::endl
+            << std::setprecision(2) << "\tAvg. Enqueued Nodes:  " << totalFrontierNodes / nof_tests << "\t\t( +"
+            << 1 + (double)(totalFrontierNodes - totalVisitedNodes) / (double)std::max(1LL, totalVisitedNodes)
+            << "x )" << std::endl
+            << std::endl;
+  fc_graph_destroy(g);
+  fc_host_graph_free(&hg);
+  return 0;
+}
