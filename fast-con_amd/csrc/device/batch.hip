@@ -888,7 +888,10 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
       }
       ctl->filt = 0;
       ctl->filters++;
-      return batch_next(ctl, false);
+      // The decision is taken again on the exact live frontiers and may pick
+      // a side that was not filtered: that side may be filtered in turn. A
+      // filtered side holds no resolved query, so at most two filters run.
+      return batch_next(ctl, true);
     }
     case kBCleanup: {
       for (int x = 0; x < 2; ++x) {
@@ -927,6 +930,10 @@ __device__ __forceinline__ void init_wave(const BatchParams<OffT>& p, int32_t* s
     t = p.qt[qi];
     live = s != t && (uint32_t)s < (uint32_t)p.V && (uint32_t)t < (uint32_t)p.V;
     if (x == 0 && s == t) p.dist[qi] = 0;
+    if (live && p.perm) {
+      s = p.perm[s];
+      t = p.perm[t];
+    }
   }
   const uint64_t lm = ballot(live);
   const int32_t u = live ? (x ? t : s) : -1;
@@ -1397,6 +1404,7 @@ static BatchParams<OffT> batch_params(fc_graph* g, BatchWorkspace* b) {
   p.qs = b->qs;
   p.qt = b->qt;
   p.dist = b->dist;
+  p.perm = g->relabeled ? g->d_perm : nullptr;
   p.mlp = g->d_mlp;
   return p;
 }

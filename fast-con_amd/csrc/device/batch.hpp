@@ -113,8 +113,9 @@ struct BatchParams {
   int64_t* hb[2][2];           // their edge bases (ascending with slot)
   uint32_t hcap;
   BatchCtl* ctl;
-  const int32_t* qs;           // queries (device)
+  const int32_t* qs;           // queries (device), caller ids
   const int32_t* qt;
+  const int32_t* perm;         // caller id -> device id (null: the same)
   int32_t* dist;               // results (device), nq entries
   const float* mlp;
   int32_t pad[2];

@@ -28,6 +28,10 @@ fc_status validate_csr32(const int32_t* off, const int32_t* col, int64_t V, int6
 fc_status validate_csr64(const int64_t* off, const int32_t* col, int64_t V, int64_t E, int64_t hub, int64_t* hubs);
 void hub_first32(const int32_t* off, const int32_t* col, int64_t V, int32_t* out);
 void hub_first64(const int64_t* off, const int32_t* col, int64_t V, int32_t* out);
+void relabel32(const int32_t* off, const int32_t* col, int64_t V, bool keep_order, int32_t* perm, int32_t* iperm,
+               int32_t* noff, int32_t* ncol);
+void relabel64(const int64_t* off, const int32_t* col, int64_t V, bool keep_order, int32_t* perm, int32_t* iperm,
+               int64_t* noff, int32_t* ncol);
 }  // namespace fc
 
 namespace fcd {
@@ -239,8 +243,28 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   CK(hipEventCreate(&g->ev1));
   CK(hipMalloc(&g->d_off, sizeof(OffT) * (size_t)(V + 1)));
   CK(hipMalloc((void**)&g->d_col, sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1)));
-  CK(hipMemcpy(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
-  if (E > 0) {
+  if (!(flags & FC_GRAPH_KEEP_IDS)) {
+    g->relabeled = true;
+    g->h_perm.resize((size_t)V);
+    g->h_iperm.resize((size_t)V);
+    std::vector<OffT> noff((size_t)V + 1);
+    std::vector<int32_t> ncol((size_t)std::max<int64_t>(E, 1));
+    if (sizeof(OffT) == 8)
+      fc::relabel64((const int64_t*)off, col, V, (flags & FC_GRAPH_KEEP_ORDER) != 0, g->h_perm.data(),
+                    g->h_iperm.data(), (int64_t*)noff.data(), ncol.data());
+    else
+      fc::relabel32((const int32_t*)off, col, V, (flags & FC_GRAPH_KEEP_ORDER) != 0, g->h_perm.data(),
+                    g->h_iperm.data(), (int32_t*)noff.data(), ncol.data());
+    CK(hipMalloc((void**)&g->d_perm, sizeof(int32_t) * (size_t)V));
+    CK(hipMalloc((void**)&g->d_iperm, sizeof(int32_t) * (size_t)V));
+    CK(hipMemcpy(g->d_perm, g->h_perm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice));
+    CK(hipMemcpy(g->d_iperm, g->h_iperm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice));
+    CK(hipMemcpy(g->d_off, noff.data(), sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
+    if (E > 0) CK(hipMemcpy(g->d_col, ncol.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+  } else {
+    CK(hipMemcpy(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
+  }
+  if (E > 0 && !g->relabeled) {
     if (flags & FC_GRAPH_KEEP_ORDER) {
       CK(hipMemcpy(g->d_col, col, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
     } else {
@@ -328,6 +352,10 @@ extern "C" fc_status fc_graph_destroy(fc_graph* g) {
   (void)hipFree(g->d_ws);
   (void)hipFree(g->d_col);
   (void)hipFree(g->d_off);
+  (void)hipFree(g->d_perm);
+  (void)hipFree(g->d_iperm);
+  (void)hipFree(g->d_lv_out);
+  (void)hipFree(g->d_par_out);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -352,19 +380,52 @@ static fc_status check_out(const fcd::VisitOut& o) {
   return FC_OK;
 }
 
+// Levels (and parents) of a BFS on a relabelled graph back in caller ids:
+// out[v] = lv[perm[v]], parents mapped through iperm. One coalesced pass.
+__global__ void unpermute_kernel(const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
+                                 const int32_t* __restrict__ lv, const int32_t* __restrict__ par,
+                                 int32_t* __restrict__ lv_out, int32_t* __restrict__ par_out, int64_t V) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+    const int32_t k = perm[v];
+    lv_out[v] = lv[k];
+    if (par) {
+      const int32_t q = par[k];
+      par_out[v] = (uint32_t)q < (uint32_t)V ? iperm[q] : q;
+    }
+  }
+}
+
 extern "C" fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32_t* parents_out, fc_stats* stats) {
   fc::clear_error();
   if (!g) return fail(FC_ERR_INVALID, "fc_bfs: null graph");
   if (src < 0 || src >= g->V) return fail(FC_ERR_INVALID, "fc_bfs: source %d out of range [0,%lld)", src, (long long)g->V);
   const bool par = parents_out != nullptr;
   const int32_t none = -1;
+  const int32_t dsrc = g->relabeled ? g->h_perm[(size_t)src] : src;
   double ms = 0;
   g->levels_valid = g->parents_valid = false;
-  fc_status st = fcd::run_visits(g, fcd::kModeBfs, 0, &src, &none, 1, par, &ms);
+  fc_status st = fcd::run_visits(g, fcd::kModeBfs, 0, &dsrc, &none, 1, par, &ms);
   if (st != FC_OK) return st;
   const fcd::VisitOut o = g->h_out[0];
   st = check_out(o);
   if (st != FC_OK) return st;
+  if (g->relabeled) {
+    // the results in caller ids are part of the BFS: timed with it
+    const size_t bytes = sizeof(int32_t) * (size_t)g->V;
+    if (!g->d_lv_out) FC_HIP(hipMalloc((void**)&g->d_lv_out, bytes));
+    if (par && !g->d_par_out) FC_HIP(hipMalloc((void**)&g->d_par_out, bytes));
+    FC_HIP(hipEventRecord(g->ev0, g->stream));
+    const int blocks = (int)std::min<int64_t>((g->V + 255) / 256, (int64_t)g->cus * 8);
+    hipLaunchKernelGGL(unpermute_kernel, dim3(blocks), dim3(256), 0, g->stream, g->d_perm, g->d_iperm, g->d_state,
+                       par ? g->d_parent : nullptr, g->d_lv_out, g->d_par_out, (int64_t)g->V);
+    FC_HIP(hipGetLastError());
+    FC_HIP(hipEventRecord(g->ev1, g->stream));
+    FC_HIP(hipEventSynchronize(g->ev1));
+    float fms = 0.f;
+    FC_HIP(hipEventElapsedTime(&fms, g->ev0, g->ev1));
+    ms += fms;
+  }
   g->levels_valid = true;
   g->parents_valid = par;
   if (stats) {
@@ -372,8 +433,10 @@ extern "C" fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32
     stats->time_ms = ms;
     add_stats(o, stats);
   }
-  if (levels_out) FC_HIP(hipMemcpy(levels_out, g->d_state, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
-  if (parents_out) FC_HIP(hipMemcpy(parents_out, g->d_parent, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  const int32_t* lv = g->relabeled ? g->d_lv_out : g->d_state;
+  const int32_t* pa = g->relabeled ? g->d_par_out : g->d_parent;
+  if (levels_out) FC_HIP(hipMemcpy(levels_out, lv, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  if (parents_out) FC_HIP(hipMemcpy(parents_out, pa, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
   return FC_OK;
 }
 
@@ -381,7 +444,8 @@ extern "C" fc_status fc_graph_levels(fc_graph* g, int32_t* levels_out) {
   fc::clear_error();
   if (!g || !levels_out) return fail(FC_ERR_INVALID, "fc_graph_levels: null argument");
   if (!g->levels_valid) return fail(FC_ERR_INVALID, "fc_graph_levels: no BFS result on the device");
-  FC_HIP(hipMemcpy(levels_out, g->d_state, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  const int32_t* lv = g->relabeled ? g->d_lv_out : g->d_state;
+  FC_HIP(hipMemcpy(levels_out, lv, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
   return FC_OK;
 }
 
@@ -389,7 +453,8 @@ extern "C" fc_status fc_graph_parents(fc_graph* g, int32_t* parents_out) {
   fc::clear_error();
   if (!g || !parents_out) return fail(FC_ERR_INVALID, "fc_graph_parents: null argument");
   if (!g->parents_valid) return fail(FC_ERR_INVALID, "fc_graph_parents: the last BFS did not record parents");
-  FC_HIP(hipMemcpy(parents_out, g->d_parent, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  const int32_t* pa = g->relabeled ? g->d_par_out : g->d_parent;
+  FC_HIP(hipMemcpy(parents_out, pa, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
   return FC_OK;
 }
 
@@ -409,8 +474,8 @@ static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
   idx.reserve(n);
   for (int32_t i = 0; i < n; ++i)
     if (s[i] != t[i]) {
-      vs.push_back(s[i]);
-      vt.push_back(t[i]);
+      vs.push_back(g->relabeled ? g->h_perm[(size_t)s[i]] : s[i]);
+      vt.push_back(g->relabeled ? g->h_perm[(size_t)t[i]] : t[i]);
       idx.push_back(i);
     }
   double ms = 0;
@@ -432,7 +497,7 @@ static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
     const int32_t i = idx[k];
     if (connected) connected[i] = o.dist >= 0 ? 1 : 0;
     if (dist) dist[i] = o.dist;
-    if (meet) meet[i] = o.meet;
+    if (meet) meet[i] = (g->relabeled && (uint32_t)o.meet < (uint32_t)g->V) ? g->h_iperm[(size_t)o.meet] : o.meet;
     if (stats) add_stats(o, stats);
   }
   return FC_OK;

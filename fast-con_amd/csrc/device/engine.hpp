@@ -56,6 +56,16 @@ struct fc_graph {
   double est_levels[3] = {12.0, 4.0, 4.0};  // running estimate per mode
   int32_t vw_fixed = 0;
 
+  // Degree relabelling (absent with FC_GRAPH_KEEP_IDS): the device graph
+  // uses new ids; perm maps a caller id to its device id, iperm back.
+  bool relabeled = false;
+  int32_t* d_perm = nullptr;
+  int32_t* d_iperm = nullptr;
+  std::vector<int32_t> h_perm, h_iperm;
+  // The last BFS's levels and parents in caller ids (relabelled graphs).
+  int32_t* d_lv_out = nullptr;
+  int32_t* d_par_out = nullptr;
+
   bool parents_valid = false;
   bool levels_valid = false;
   float* d_mlp = nullptr;           // scheduler parameters (device)

@@ -244,6 +244,51 @@ static void hub_first_t(const OffT* off, const int32_t* col, int64_t V, int32_t*
 void hub_first32(const int32_t* off, const int32_t* col, int64_t V, int32_t* out) { hub_first_t(off, col, V, out); }
 void hub_first64(const int64_t* off, const int32_t* col, int64_t V, int32_t* out) { hub_first_t(off, col, V, out); }
 
+// Degree relabelling: new ids in order of decreasing degree, ties by old id
+// (a stable counting sort on degree), so that the per-vertex records of the
+// hubs, which most edges lead to, share cache lines. perm[old] = new,
+// iperm[new] = old; noff/ncol is the relabelled CSR. Each list is sorted by
+// new id unless keep_order, which is the hub-first order above (decreasing
+// degree, ties by old id) expressed in new ids.
+template <typename OffT>
+static void relabel_t(const OffT* off, const int32_t* col, int64_t V, bool keep_order, int32_t* perm,
+                      int32_t* iperm, OffT* noff, int32_t* ncol) {
+  int64_t maxd = 0;
+#pragma omp parallel for reduction(max : maxd)
+  for (int64_t v = 0; v < V; ++v) maxd = std::max<int64_t>(maxd, (int64_t)off[v + 1] - (int64_t)off[v]);
+  // bucket b holds degree maxd - b, so buckets run by decreasing degree
+  std::vector<int64_t> start((size_t)maxd + 2, 0);
+  for (int64_t v = 0; v < V; ++v) start[(size_t)(maxd - ((int64_t)off[v + 1] - (int64_t)off[v])) + 1]++;
+  for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+  for (int64_t v = 0; v < V; ++v) {
+    const int64_t k = start[(size_t)(maxd - ((int64_t)off[v + 1] - (int64_t)off[v]))]++;
+    iperm[k] = (int32_t)v;
+  }
+#pragma omp parallel for
+  for (int64_t k = 0; k < V; ++k) perm[iperm[k]] = (int32_t)k;
+  noff[0] = 0;
+  for (int64_t k = 0; k < V; ++k) {
+    const int32_t v = iperm[k];
+    noff[k + 1] = noff[k] + (off[v + 1] - off[v]);
+  }
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t k = 0; k < V; ++k) {
+    const int32_t v = iperm[k];
+    const int64_t b = (int64_t)off[v], e = (int64_t)off[v + 1], nb = (int64_t)noff[k];
+    for (int64_t j = b; j < e; ++j) ncol[nb + (j - b)] = perm[col[j]];
+    if (!keep_order) std::sort(ncol + nb, ncol + nb + (e - b));
+  }
+}
+
+void relabel32(const int32_t* off, const int32_t* col, int64_t V, bool keep_order, int32_t* perm, int32_t* iperm,
+               int32_t* noff, int32_t* ncol) {
+  relabel_t(off, col, V, keep_order, perm, iperm, noff, ncol);
+}
+void relabel64(const int64_t* off, const int32_t* col, int64_t V, bool keep_order, int32_t* perm, int32_t* iperm,
+               int64_t* noff, int32_t* ncol) {
+  relabel_t(off, col, V, keep_order, perm, iperm, noff, ncol);
+}
+
 fc_status validate_csr32(const int32_t* off, const int32_t* col, int64_t V, int64_t E, int64_t hub, int64_t* hubs) {
   return validate_csr_t(off, col, V, E, hub, hubs);
 }

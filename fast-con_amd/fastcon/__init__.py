@@ -29,6 +29,7 @@ FC_INF = -1
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_IO, ERR_FORMAT, ERR_CAPACITY, ERR_NOGPU, ERR_TIMEOUT, ERR_UNSUPPORTED = range(10)
 POLICY_PLAIN, POLICY_HEURISTIC, POLICY_LEARNED = 0, 1, 2
 GRAPH_KEEP_ORDER = 1
+GRAPH_KEEP_IDS = 2
 DIR_FILE, DIR_DIRECTED, DIR_UNDIRECTED = 0, 1, 2
 SCHED_FEATURES, SCHED_HIDDEN, SCHED_ACTIONS = 12, 32, 6
 SCHED_PARAMS = 12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6
@@ -249,12 +250,14 @@ class StResult:
 class Graph:
     """A graph resident on the GPU (library-owned device copy + workspace)."""
 
-    def __init__(self, hg: HostGraph, force64: bool = False, keep_order: bool = False):
+    def __init__(self, hg: HostGraph, force64: bool = False, keep_order: bool = False, keep_ids: bool = False):
         """keep_order: keep the caller's adjacency order on the device
-        (default: neighbours by decreasing degree, see fc_graph_create)."""
+        (default: neighbours by decreasing degree); keep_ids: keep the
+        caller's vertex numbering on the device (default: renumbered by
+        decreasing degree). Calls take and return caller ids either way."""
         self.V, self.E = hg.V, hg.E
         self._h = C.c_void_p()
-        flags = GRAPH_KEEP_ORDER if keep_order else 0
+        flags = (GRAPH_KEEP_ORDER if keep_order else 0) | (GRAPH_KEEP_IDS if keep_ids else 0)
         if hg.E < 2**31 - 1 and not force64:
             off32 = hg.offsets.astype(np.int32)
             _check(_lib.fc_graph_create_csr32(_p32(off32), _p32(hg.columns), hg.V, hg.E, flags, C.byref(self._h)))
@@ -293,6 +296,11 @@ class Graph:
         lv = np.empty(self.V, np.int32)
         _check(_lib.fc_graph_levels(self._h, _p32(lv)))
         return lv
+
+    def parents(self) -> np.ndarray:
+        pa = np.empty(self.V, np.int32)
+        _check(_lib.fc_graph_parents(self._h, _p32(pa)))
+        return pa
 
     def stcon(self, s: int, t: int):
         conn, dist, meet = C.c_int32(), C.c_int32(), C.c_int32()

@@ -138,12 +138,16 @@ typedef struct fc_graph fc_graph;
 
 /* Upload a CSR graph; the library keeps device copies and allocates all
  * per-query workspace once. Replaces cudaGraph::cudaGraph
- * (BFS-4K/src/device/cudaGraph.cu:7-36). By default the device copy lists
- * each vertex's neighbours by decreasing degree (ties by id), which makes
- * bottom-up levels stop early; levels, verdicts and distances are the same
- * for any adjacency order. flags: FC_GRAPH_KEEP_ORDER keeps the caller's
- * order (then BFS parents follow the caller's order). */
+ * (BFS-4K/src/device/cudaGraph.cu:7-36). By default the device copy
+ * numbers the vertices by decreasing degree (ties by id), so that the
+ * per-vertex state of the hubs shares cache lines, and lists each vertex's
+ * neighbours by decreasing degree, which makes bottom-up levels stop early.
+ * Every call takes and returns the caller's vertex ids; levels, verdicts and
+ * distances do not depend on either choice. flags: FC_GRAPH_KEEP_ORDER keeps
+ * the caller's adjacency order (then BFS parents follow it);
+ * FC_GRAPH_KEEP_IDS keeps the caller's numbering on the device. */
 #define FC_GRAPH_KEEP_ORDER 1u
+#define FC_GRAPH_KEEP_IDS 2u
 fc_status fc_graph_create_csr32(const int32_t* off, const int32_t* col, int32_t V, int64_t E,
                                 uint32_t flags, fc_graph** out);
 fc_status fc_graph_create_csr64(const int64_t* off, const int32_t* col, int64_t V, int64_t E,

@@ -189,16 +189,27 @@ def test_many_queries_reuse_and_errors(fc, orc):
         assert np.array_equal(lv, d)
 
 
-def test_adjacency_order_does_not_change_answers(fc, orc):
-    """The device copy's hub-first adjacency order and the caller's order give
-    the same levels and distances, and valid BFS trees either way."""
+@pytest.mark.parametrize("keep_ids", [False, True])
+@pytest.mark.parametrize("keep_order", [False, True])
+def test_layout_does_not_change_answers(fc, orc, keep_order, keep_ids):
+    """The device copy's degree numbering and hub-first adjacency order, and
+    the caller's numbering and order, give the same levels and distances in
+    caller ids, valid BFS trees and meeting vertices on shortest paths."""
     hg = fc.gen_rmat(15, 16, seed=12)
     roots = hg.sample_roots(64, 13)
     s, t = roots[:32], roots[32:]
     exp = np.array([orc.stcon(hg.offsets, hg.columns, a, b)[1] for a, b in zip(s, t)], np.int32)
-    for keep in (False, True):
-        with fc.Graph(hg, keep_order=keep) as g:
-            say(f"adjacency order keep={keep}")
-            _bfs_check(fc, orc, f"rmat15 keep={keep}", hg, g, roots[:4])
-            assert np.array_equal(g.stcon_many(s, t).dist, exp)
-            assert np.array_equal(g.stcon_batch(s, t).dist, exp)
+    with fc.Graph(hg, keep_order=keep_order, keep_ids=keep_ids) as g:
+        say(f"layout keep_order={keep_order} keep_ids={keep_ids}")
+        _bfs_check(fc, orc, f"rmat15 {keep_order} {keep_ids}", hg, g, roots[:4])
+        assert np.array_equal(g.stcon_many(s, t).dist, exp)
+        assert np.array_equal(g.stcon_batch(s, t).dist, exp)
+        lv, pa, _ = g.bfs(int(s[0]), parents=True)
+        assert np.array_equal(g.levels(), lv) and np.array_equal(g.parents(), pa)
+        for k in range(4):
+            c, d, meet, _ = g.stcon(int(s[k]), int(t[k]))
+            assert d == exp[k]
+            if c and d > 0:
+                ds, _, _ = orc.bfs(hg.offsets, hg.columns, int(s[k]))
+                dt, _, _ = orc.bfs(hg.offsets, hg.columns, int(t[k]))
+                assert ds[meet] >= 0 and dt[meet] >= 0 and ds[meet] + dt[meet] == d
