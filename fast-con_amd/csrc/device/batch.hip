@@ -393,34 +393,20 @@ __device__ __forceinline__ void expand_side(const BatchParams<OffT>& p, int x, i
   expand_hubs(p, x, par, ctl->hub[x][par], act, chunk, gwave, nwaves, a_cnt, e_cnt);
 }
 
-// Scan of a per-32-vertex byte array (touched or dirty groups) in vertex
-// order: the wave reads 64 group bytes, then handles the non-zero groups two
-// at a time, lanes 0..31 on the first and 32..63 on the second, so that the
-// per-vertex records of a group are read and written in address order. Each
-// group belongs to one wave of the launch. fn(u, valid, g0, g1) runs with
-// every lane active; g0/g1 are the groups of the low and high half-wave.
-template <typename Fn>
-__device__ __forceinline__ void scan_groups(uint8_t* marks, int64_t V, int gwave, int nwaves, Fn&& fn) {
-  const int lane = lane_id();
-  const int64_t groups = (V + 31) / 32;
-  for (int64_t gb = (int64_t)gwave * kWave; gb < groups; gb += (int64_t)nwaves * kWave) {
-    const int64_t gi = gb + lane;
-    const bool marked = gi < groups && marks[gi] != 0;
-    if (marked) marks[gi] = 0;
-    uint64_t nz = ballot(marked);
-    while (nz) {
-      const int k0 = __builtin_ctzll(nz);
-      nz &= nz - 1;
-      int k1 = -1;
-      if (nz) {
-        k1 = __builtin_ctzll(nz);
-        nz &= nz - 1;
-      }
-      const int mine = lane < 32 ? k0 : k1;
-      const int64_t u = (gb + (mine < 0 ? 0 : mine)) * 32 + (lane & 31);
-      fn((int32_t)u, mine >= 0 && u < V, gb + k0, k1 >= 0 ? gb + k1 : (int64_t)-1);
-    }
-  }
+// Groups of 32 vertices are dealt to the waves of a launch round-robin:
+// wave w owns groups w, w + nwaves, w + 2 nwaves, ... With vertices
+// numbered by decreasing degree the touched groups crowd at low ids, and
+// this spreads them over every wave. A wave reads the marks of 64 of its
+// groups at a time; slot k of such a read is group (jb + k) * nwaves + w.
+struct GroupDeal {
+  int64_t jb;
+  int nwaves, gwave;
+  __device__ __forceinline__ int64_t operator()(int k) const { return (jb + k) * (int64_t)nwaves + gwave; }
+};
+
+// Number of groups owned by wave gwave.
+__device__ __forceinline__ int64_t owned_groups(int64_t groups, int gwave, int nwaves) {
+  return groups > gwave ? (groups - gwave + nwaves - 1) / nwaves : 0;
 }
 
 // Mark dirty vertices (visited words became non-zero this wave) in the dirty
@@ -436,8 +422,9 @@ __device__ __forceinline__ void mark_dirty(const BatchParams<OffT>& p, bool dirt
 
 // Commit: every vertex of a touched group folds its N words into its visited
 // words, and newly reached vertices join the next frontiers. The wave reads
-// 64 group marks, then handles the marked groups two at a time (lanes 0..31
-// and 32..63, records read and written in address order); the records and
+// the marks of 64 of its groups (dealt round-robin), then handles the marked
+// groups two at a time (lanes 0..31 and 32..63, each half-wave reading and
+// writing one group's records in address order); the records and
 // offsets of the next pair are loaded before the current pair is processed,
 // so each pass costs about one memory round trip. Each group belongs to one
 // wave of the launch, which therefore updates its dirty-bitmap word without
@@ -451,11 +438,12 @@ struct CommitLoad {
 };
 
 template <typename OffT>
-__device__ __forceinline__ CommitLoad<OffT> commit_load(const BatchParams<OffT>& p, int64_t gb, int k0, int k1) {
+__device__ __forceinline__ CommitLoad<OffT> commit_load(const BatchParams<OffT>& p, const GroupDeal& gd, int k0,
+                                                        int k1) {
   const int lane = lane_id();
   const int mine = lane < 32 ? k0 : k1;
   CommitLoad<OffT> c;
-  const int64_t u = (gb + (mine < 0 ? 0 : mine)) * 32 + (lane & 31);
+  const int64_t u = gd(mine < 0 ? 0 : mine) * 32 + (lane & 31);
   c.u = (int32_t)u;
   c.valid = mine >= 0 && u < p.V;
   c.a = c.b = make_ulonglong2(0ull, 0ull);
@@ -486,20 +474,21 @@ template <typename OffT>
 __device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends& ap, unsigned long long act,
                                             const int* nxt, int gwave, int nwaves, unsigned long long* red) {
   const int lane = lane_id();
-  const int64_t groups = ((int64_t)p.V + 31) / 32;
-  for (int64_t gb = (int64_t)gwave * kWave; gb < groups; gb += (int64_t)nwaves * kWave) {
-    const int64_t gi = gb + lane;
-    const bool marked = gi < groups && p.tm[gi] != 0;
+  const int64_t per = owned_groups(((int64_t)p.V + 31) / 32, gwave, nwaves);
+  for (int64_t jb = 0; jb < per; jb += kWave) {
+    const GroupDeal gd{jb, nwaves, gwave};
+    const int64_t gi = gd(lane);
+    const bool marked = jb + lane < per && p.tm[gi] != 0;
     if (marked) p.tm[gi] = 0;
     uint64_t nz = ballot(marked);
     int k0, k1;
     if (!take_pair(nz, k0, k1)) continue;
-    CommitLoad<OffT> cur = commit_load(p, gb, k0, k1);
+    CommitLoad<OffT> cur = commit_load(p, gd, k0, k1);
     for (;;) {
       int n0 = -1, n1 = -1;
       const bool more = take_pair(nz, n0, n1);
       CommitLoad<OffT> nx;
-      if (more) nx = commit_load(p, gb, n0, n1);
+      if (more) nx = commit_load(p, gd, n0, n1);
       unsigned long long nS = 0, nT = 0, vS = 0, vT = 0;
       int64_t deg = 0;
       const int32_t u = cur.u;
@@ -526,7 +515,7 @@ __device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends&
       }
       push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
       push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
-      mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), gb + k0, k1 >= 0 ? gb + k1 : (int64_t)-1);
+      mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), gd(k0), k1 >= 0 ? gd(k1) : (int64_t)-1);
       if (!more) break;
       cur = nx;
       k0 = n0;
@@ -1075,10 +1064,11 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       // the records and all four mask arrays of the dirty vertices in
       // address order.
       const int lane = lane_id();
-      const int64_t words = ((int64_t)p.V + 31) / 32;
-      for (int64_t wb = (int64_t)gwave * kWave; wb < words; wb += (int64_t)nwaves * kWave) {
-        const int64_t wi = wb + lane;
-        const uint32_t word = wi < words ? p.db[wi] : 0u;
+      const int64_t per = owned_groups(((int64_t)p.V + 31) / 32, gwave, nwaves);
+      for (int64_t jb = 0; jb < per; jb += kWave) {
+        const GroupDeal gd{jb, nwaves, gwave};
+        const int64_t wi = gd(lane);
+        const uint32_t word = jb + lane < per ? p.db[wi] : 0u;
         if (word) p.db[wi] = 0u;
         uint64_t nz = ballot(word != 0);
         while (nz) {
@@ -1091,7 +1081,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
           }
           const int mine = lane < 32 ? k0 : k1;
           const uint32_t wv = (uint32_t)__shfl((int)word, mine < 0 ? 0 : mine);
-          const int64_t u = (wb + (mine < 0 ? 0 : mine)) * 32 + (lane & 31);
+          const int64_t u = gd(mine < 0 ? 0 : mine) * 32 + (lane & 31);
           if (mine >= 0 && ((wv >> (lane & 31)) & 1u) && u < p.V) {
             ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * u);
             r[0] = make_ulonglong2(0ull, 0ull);
