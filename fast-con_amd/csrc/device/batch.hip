@@ -634,11 +634,12 @@ __device__ __forceinline__ void filter_side(const BatchParams<OffT>& p, Appends&
 }
 
 // Features of the pending step for the learned scheduler (12 values in
-// about [0, 1], scale-free except f4): per side the live frontier's size and
-// degree sum relative to the graph's (on a log scale), its mean degree
-// relative to the graph's, the side's level and the fraction of (vertex,
-// query) pairs it has visited; the graph's edge count; the live-query
-// fraction.
+// about [0, 1]): per side the live frontier's size and degree sum relative
+// to the graph's (on a log scale), its mean degree relative to the graph's,
+// the side's level and the fraction of (vertex, query) pairs it has visited;
+// the graph's mean excess degree sum d^2 / sum d (the expected degree of a
+// vertex reached over an edge, which sets how fast frontiers grow, on a log
+// scale); the live-query fraction.
 __device__ __forceinline__ void batch_features(BatchCtl* ctl, const uint64_t* mf, int64_t V, int64_t E) {
   auto lg = [](double x) { return log2(x + 1.0); };
   float* f = ctl->feat;
@@ -656,7 +657,7 @@ __device__ __forceinline__ void batch_features(BatchCtl* ctl, const uint64_t* mf
   f[1] = (float)(lg((double)mf[0]) / le);
   f[2] = (float)(lg(n[1]) / lv);
   f[3] = (float)(lg((double)mf[1]) / le);
-  f[4] = (float)(le / 40.0);
+  f[4] = ctl->f_branch;
   f[5] = (float)((double)ctl->vis_pairs[0] / pairs);
   f[6] = (float)(ctl->lvl[0] < 64 ? ctl->lvl[0] : 64) / 64.f;
   f[7] = (float)(ctl->lvl[1] < 64 ? ctl->lvl[1] : 64) / 64.f;
@@ -1122,7 +1123,19 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   if (!sh.last || threadIdx.x >= kWave) return;
   // Wave 0 of the last block finalises: slot sums, the state update (lane
   // 0), and, when a step must be chosen, the decision (the learned policy
-  // evaluates its network across the wave).
+  // evaluates its network across the wave). The network's weights are
+  // fetched first, so that their loads overlap the finalisation, and are
+  // staged in LDS (the appends' stage is free now).
+  constexpr int kPer = (kMlpParams + kWave - 1) / kWave;
+  const bool mlp_on = ctl->policy == 2 && p.mlp != nullptr;
+  float wreg[kPer];
+  if (mlp_on) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = k * kWave + lane_id();
+      wreg[k] = i < kMlpParams ? p.mlp[i] : 0.f;
+    }
+  }
   int need = kFinDone;
   {
     const RedTot tot = collect_slots(p.red);
@@ -1148,10 +1161,18 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
     const bool forced = phase == kBDecide && ctl->force_action >= 0;
     const bool pause = phase != kBDecide && ctl->trace &&
                        ctl->decisions % (ctl->trace_every > 0 ? ctl->trace_every : 1) == 0;
-    if (!forced && !pause && ctl->policy == 2 && p.mlp) {
+    if (!forced && !pause && mlp_on) {
+      float* s_mlp = reinterpret_cast<float*>(&s_stage[0][0]);
+      static_assert(kMlpParams * sizeof(float) <= sizeof(s_stage), "network does not fit the LDS stage");
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int i = k * kWave + lane_id();
+        if (i < kMlpParams) s_mlp[mlp_t_index(i)] = wreg[k];
+      }
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < 12; ++k) f[k] = __shfl(f[k], 0);
-      a = mlp_pick_wave(p.mlp, f, a, 1e-3f * (float)ctl->learn_margin_milli);
+      a = mlp_pick_wave_t(s_mlp, f, a, 1e-3f * (float)ctl->learn_margin_milli);
     }
     if (threadIdx.x == 0) {
       if (pause) {
@@ -1435,6 +1456,7 @@ fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int
   init.trace = trace;
   init.trace_every = every;
   init.learn_margin_milli = (int32_t)std::lround((margin < 0.f ? kDefaultLearnMargin : margin) * 1000.f);
+  init.f_branch = g->f_branch;
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error

@@ -66,7 +66,7 @@ struct BatchCtl {
   int32_t heur_action;     // the heuristic's action at the pause
   int32_t learn_margin_milli;  // learned policy: keep the heuristic's action unless the
                                // network predicts a log-cost lower by this (x1e-3)
-  int32_t pad0;
+  float f_branch;          // feature 4: log2(1 + sum d^2 / sum d) / 16 of the graph
   int64_t vis_pairs[2];    // (vertex, query) pairs visited per side in this wave
   int32_t live0;           // live queries at the start of the wave
   int32_t pad1;

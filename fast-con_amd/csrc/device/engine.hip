@@ -229,6 +229,14 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
     fc_graph_destroy(g);
     return s;
   };
+  {
+    double d2 = 0.0;
+    for (int64_t v = 0; v < V; ++v) {
+      const double d = (double)((int64_t)off[v + 1] - (int64_t)off[v]);
+      d2 += d * d;
+    }
+    g->f_branch = (float)(std::log2(1.0 + (E > 0 ? d2 / (double)E : 0.0)) / 16.0);
+  }
 #define CK(call)                                            \
   do {                                                      \
     hipError_t _e = (call);                                 \

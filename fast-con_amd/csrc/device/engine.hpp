@@ -26,6 +26,7 @@ struct fc_graph {
   int cus = 0;
   int blocks = 0;               // cooperative grid of the visit kernel
   int32_t hub_threshold = 4096;
+  float f_branch = 0.f;         // scheduler feature: log2(1 + sum d^2 / sum d) / 16
   uint32_t hcap = 1;
   uint32_t qcap = 1;
 

@@ -13,6 +13,7 @@ Reference interfaces mirrored (paths relative to the reference checkout):
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 from dataclasses import dataclass
 
@@ -321,8 +322,14 @@ class Graph:
                                             C.byref(st)))
         return StResult(conn, dist, st.as_dict())
 
+    # margin of the learned policy used when a call gives none (negative:
+    # the library's default); load_default_sched_params sets the shipped one
+    learn_margin = -1.0
+
     def stcon_batch(self, s, t, policy: int = POLICY_HEURISTIC, vw: int = 0, tile: int = 0, use_graph: int = 1,
-                    words: int = 1, learn_margin: float = -1.0, lanes: int = 0) -> StResult:
+                    words: int = 1, learn_margin: float = None, lanes: int = 0) -> StResult:
+        if learn_margin is None:
+            learn_margin = self.learn_margin
         s = np.ascontiguousarray(s, np.int32)
         t = np.ascontiguousarray(t, np.int32)
         conn = np.empty(len(s), np.int32)
@@ -406,11 +413,19 @@ DEFAULT_SCHED_PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), 
 
 
 def load_default_sched_params(g: "Graph", path: str = DEFAULT_SCHED_PARAMS) -> bool:
-    """Set the shipped scheduler parameters on g; False when there are none."""
+    """Set the shipped scheduler parameters on g, and the margin chosen with
+    them (sched_report.json beside them) as g's default; False when there
+    are none."""
     if not os.path.exists(path):
         return False
     w = np.load(path, allow_pickle=False)
     if w.dtype != np.float32 or w.shape != (SCHED_PARAMS,):
         raise ValueError(f"{path}: expected {SCHED_PARAMS} float32 parameters")
     g.sched_set_params(w)
+    rep = os.path.join(os.path.dirname(path), "sched_report.json")
+    if os.path.exists(rep):
+        with open(rep) as fh:
+            m = json.load(fh).get("margin")
+        if isinstance(m, (int, float)) and m >= 0:
+            g.learn_margin = float(m)
     return True

@@ -3,17 +3,21 @@
     python -m fastcon.train_sched [--out DIR] [--quick]
 
 1. Trace: on a seeded family of graphs (undirected R-MAT with the initiator
-   a varied around 0.57, Erdős–Rényi with mean degree 8, 2-D grids with 2% of
-   the edges deleted) the engine runs seeded s–t batches along the heuristic
+   a varied around 0.57 and edge factors 8 and 16, Erdős–Rényi with mean
+   degrees 6 to 16, 2-D grids with 2% of the edges deleted) the engine runs seeded s–t batches along the heuristic
    schedule and, at every traced step decision, times each of the six
    actions (side s / t / both, top-down / bottom-up) from that state to the
    end of the 64-query wave (fc_sched_trace).
+   Each state is timed --repeats times (the same decisions each time) and
+   the median cost is the label.
 2. Train: the 12 -> 32 -> 32 -> 6 network is fitted on the GPU to the
    centred log-costs by the engine's own forward/backward/Adam kernels
    (fc_sched_train). Then the training graphs are traced again along the
    learned schedule itself, on new queries, and the network is refitted on
    both sets of rows (one round of dataset aggregation, so that it also sees
    the states its own choices lead to).
+   The margin by which the network must beat the heuristic's action is
+   chosen on validation graphs (total regret over a grid of margins).
 3. Report, on the training rows and on held-out graphs and queries:
    regret (cost of the chosen action over the best action, per traced step)
    of the network, the heuristic and the plain both-sides schedule; and the
@@ -41,9 +45,9 @@ def log(*a):
     print(*a, flush=True)
 
 
-def rmat(scale, a, seed):
+def rmat(scale, a, seed, ef=16):
     b = (1.0 - a - 0.05) / 2.0
-    return fc.gen_rmat(scale, 16, a, b, b, seed=seed)
+    return fc.gen_rmat(scale, ef, a, b, b, seed=seed)
 
 
 def family(quick):
@@ -52,34 +56,49 @@ def family(quick):
     if quick:
         train = [("rmat16_a0.57", lambda: rmat(16, 0.57, 5), 256, 1, 400),
                  ("grid128", lambda: fc.gen_grid(128, 128, 0.02, 7), 128, 8, 120)]
+        valid = [("rmat16_a0.60", lambda: rmat(16, 0.60, 10), 256, 1, 400)]
         held = [("rmat16_a0.50", lambda: rmat(16, 0.50, 9), 256, 1, 400)]
-        return train, held
+        return train, valid, held
     train = [("rmat20_a0.45", lambda: rmat(20, 0.45, 20), 1024, 1, 800),
              ("rmat20_a0.57", lambda: rmat(20, 0.57, 21), 1024, 1, 800),
              ("rmat20_a0.65", lambda: rmat(20, 0.65, 22), 1024, 1, 800),
              ("rmat21_a0.52", lambda: rmat(21, 0.52, 23), 1024, 1, 800),
              ("rmat22_a0.60", lambda: rmat(22, 0.60, 26), 1024, 1, 800),
              ("rmat23_a0.55", lambda: rmat(23, 0.55, 27), 512, 1, 800),
+             ("rmat21_a0.57_ef8", lambda: rmat(21, 0.57, 34, ef=8), 1024, 1, 800),
              ("er2M_d8", lambda: fc.gen_er(1 << 21, 8.0, 24), 1024, 1, 800),
              ("er4M_d6", lambda: fc.gen_er(1 << 22, 6.0, 28), 512, 1, 800),
+             ("er2M_d16", lambda: fc.gen_er(1 << 21, 16.0, 35), 1024, 1, 800),
              ("grid512", lambda: fc.gen_grid(512, 512, 0.02, 25), 256, 16, 300),
              ("grid1024", lambda: fc.gen_grid(1024, 1024, 0.02, 29), 128, 32, 200)]
+    valid = [("rmat21_a0.57", lambda: rmat(21, 0.57, 41), 1024, 1, 800),
+             ("er4M_d8", lambda: fc.gen_er(1 << 22, 8.0, 42), 512, 1, 600)]
     held = [("rmat22_a0.57", lambda: rmat(22, 0.57, 22), 1024, 1, 1200),
             ("rmat21_a0.62", lambda: rmat(21, 0.62, 31), 512, 1, 800),
             ("er8M_d8", lambda: fc.gen_er(1 << 23, 8.0, 32), 512, 1, 600),
             ("grid2048", lambda: fc.gen_grid(2048, 2048, 0.02, 33), 256, 64, 200)]
-    return train, held
+    return train, valid, held
 
 
-def trace_graph(name, g, hg, pairs, every, cap, seed, policy=fc.POLICY_HEURISTIC):
+def trace_graph(name, g, hg, pairs, every, cap, seed, policy=fc.POLICY_HEURISTIC, repeats=1):
     t0 = time.time()
     r = hg.sample_roots(2 * pairs, seed)
     s, t = r[:pairs], r[pairs:]
-    feats, cost, heur, wave, dist = g.sched_trace(s, t, cap=cap, every=every, policy=policy)
     ref = g.stcon_batch(s, t, policy=fc.POLICY_PLAIN).dist
-    assert np.array_equal(dist, ref), f"{name}: traced distances differ from the plain schedule"
-    log(f"  {name}: V={hg.V} E={hg.E} pairs={pairs} rows={len(feats)} ({time.time() - t0:.1f}s)")
-    return s, t, feats, cost, heur
+    costs = []
+    for k in range(repeats):
+        feats, cost, heur, wave, dist = g.sched_trace(s, t, cap=cap, every=every, policy=policy)
+        assert np.array_equal(dist, ref), f"{name}: traced distances differ from the plain schedule"
+        if k == 0:
+            f0, h0 = feats, heur
+        else:  # the same schedule visits the same states
+            assert np.array_equal(feats, f0) and np.array_equal(heur, h0), f"{name}: trace {k} took other steps"
+        costs.append(cost)
+    cost = np.median(np.stack(costs), axis=0)
+    spread = float(np.median(np.ptp(np.stack(costs), axis=0) / cost.clip(min=1e-9))) if repeats > 1 else 0.0
+    log(f"  {name}: V={hg.V} E={hg.E} pairs={pairs} rows={len(f0)} x{repeats} "
+        f"(median relative spread {spread:.3f}; {time.time() - t0:.1f}s)")
+    return s, t, f0, cost, h0
 
 
 def open_graph(build, seed):
@@ -130,16 +149,16 @@ def main():
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--margin", type=float, default=0.1)
+    ap.add_argument("--repeats", type=int, default=3, help="timed rollouts per traced state (median)")
     args = ap.parse_args()
-    train, held = family(args.quick)
+    train, valid, held = family(args.quick)
 
     log("trace (training family, heuristic schedule)")
     F, Cst, Hh = [], [], []
     graphs = []
     for i, (name, build, pairs, every, cap) in enumerate(train):
         g, hg = open_graph(build, 50 + i)
-        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 100 + i)
+        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 100 + i, repeats=args.repeats)
         F.append(f), Cst.append(c), Hh.append(h)
         graphs.append((name, g, hg, pairs, every, cap))
     g0 = graphs[0][1]
@@ -150,7 +169,8 @@ def main():
     log("trace (training family, learned schedule)")
     for i, (name, g, hg, pairs, every, cap) in enumerate(graphs):
         g.sched_set_params(w1)
-        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 150 + i, policy=fc.POLICY_LEARNED)
+        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 150 + i, policy=fc.POLICY_LEARNED,
+                                    repeats=args.repeats)
         F.append(f), Cst.append(c), Hh.append(h)
     F, Cst, Hh = np.concatenate(F), np.concatenate(Cst), np.concatenate(Hh)
 
@@ -160,18 +180,36 @@ def main():
     w = g0.sched_get_params()
     log(f"  final loss {loss:.4f} ({time.time() - t0:.1f}s)")
 
+    # margin: the one with the lowest total regret on the validation graphs
+    log("validation graphs (margin choice)")
+    VF, VC, VH = [], [], []
+    for i, (name, build, pairs, every, cap) in enumerate(valid):
+        g, hg = open_graph(build, 350 + i)
+        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 400 + i, repeats=args.repeats)
+        VF.append(f), VC.append(c), VH.append(h)
+        g.close()
+    VF, VC, VH = np.concatenate(VF), np.concatenate(VC), np.concatenate(VH)
+    vpred = g0.sched_forward(VF)
+    margins = [0.0, 0.02, 0.05, 0.1, 0.15, 0.2, 0.3, 0.5, 1.0]
+    vreg = {m: regret(VC, pick_actions(vpred, VH, m))[1] for m in margins}
+    margin = min(margins, key=lambda m: (round(vreg[m], 4), -m))
+    log("  total regret by margin " + json.dumps({str(m): round(v, 4) for m, v in vreg.items()}) +
+        f"; heuristic {regret(VC, VH)[1]:.4f}; chosen {margin}")
+
     def report_rows(feats, cost, heur):
         pred = g0.sched_forward(feats)
         rows = {"rows": int(len(feats))}
-        for name, p in (("learned", pick_actions(pred, heur, args.margin)), ("network_argmin", np.argmin(pred, 1)),
+        for name, p in (("learned", pick_actions(pred, heur, margin)), ("network_argmin", np.argmin(pred, 1)),
                         ("heuristic", heur), ("plain", np.full(len(cost), 2))):
             mr, tr = regret(cost, p)
             rows[name] = {"mean_regret": round(mr, 4), "total_regret": round(tr, 4)}
         rows["best_fixed_action"] = int(np.argmin(cost.sum(axis=0)))
         return rows
 
-    report = {"train": report_rows(F, Cst, Hh), "held_out": {}, "end_to_end": {}, "loss": loss,
-              "epochs": args.epochs, "lr": args.lr, "margin": args.margin}
+    report = {"train": report_rows(F, Cst, Hh), "validation": report_rows(VF, VC, VH), "held_out": {},
+              "end_to_end": {}, "loss": loss, "epochs": args.epochs, "lr": args.lr, "margin": margin,
+              "validation_total_regret_by_margin": {str(m): round(v, 4) for m, v in vreg.items()},
+              "repeats": args.repeats}
     log("train-set regret " + json.dumps(report["train"]))
     for name, g, hg, pairs, every, cap in graphs:
         if g is not g0:
@@ -179,10 +217,10 @@ def main():
     log("held-out graphs")
     for i, (name, build, pairs, every, cap) in enumerate(held):
         g, hg = open_graph(build, 250 + i)
-        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 200 + i)
+        s, t, f, c, h = trace_graph(name, g, hg, pairs, every, cap, 200 + i, repeats=args.repeats)
         report["held_out"][name] = report_rows(f, c, h)
         r2 = hg.sample_roots(2 * pairs, 300 + i)
-        report["end_to_end"][name] = end_to_end(g, r2[:pairs], r2[pairs:], w, args.margin)
+        report["end_to_end"][name] = end_to_end(g, r2[:pairs], r2[pairs:], w, margin)
         log(f"  {name}: regret {json.dumps(report['held_out'][name])}")
         log(f"  {name}: end to end {json.dumps(report['end_to_end'][name])}")
         g.close()
