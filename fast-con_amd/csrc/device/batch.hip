@@ -1124,16 +1124,22 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   // Wave 0 of the last block finalises: slot sums, the state update (lane
   // 0), and, when a step must be chosen, the decision (the learned policy
   // evaluates its network across the wave). The network's weights are
-  // fetched first, so that their loads overlap the finalisation, and are
-  // staged in LDS (the appends' stage is free now).
+  // copied from global memory straight into LDS (the appends' stage is free
+  // now) first, so that the copy overlaps the finalisation.
   constexpr int kPer = (kMlpParams + kWave - 1) / kWave;
+  static_assert(kPer * kWave * sizeof(float) <= sizeof(s_stage), "network does not fit the LDS stage");
   const bool mlp_on = ctl->policy == 2 && p.mlp != nullptr;
-  float wreg[kPer];
+  float* s_mlp = reinterpret_cast<float*>(&s_stage[0][0]);
   if (mlp_on) {
+    typedef __attribute__((address_space(1))) void* gptr;
+    typedef __attribute__((address_space(3))) void* lptr;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
+      // lane l writes word k * kWave + l of s_mlp; lanes past the end re-read
+      // the last weight into the unused tail of the stage
       const int i = k * kWave + lane_id();
-      wreg[k] = i < kMlpParams ? p.mlp[i] : 0.f;
+      const float* src = p.mlp + (i < kMlpParams ? i : kMlpParams - 1);
+      __builtin_amdgcn_global_load_lds((gptr)src, (lptr)(s_mlp + k * kWave), 4, 0, 0);
     }
   }
   int need = kFinDone;
@@ -1162,13 +1168,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
     const bool pause = phase != kBDecide && ctl->trace &&
                        ctl->decisions % (ctl->trace_every > 0 ? ctl->trace_every : 1) == 0;
     if (!forced && !pause && mlp_on) {
-      float* s_mlp = reinterpret_cast<float*>(&s_stage[0][0]);
-      static_assert(kMlpParams * sizeof(float) <= sizeof(s_stage), "network does not fit the LDS stage");
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const int i = k * kWave + lane_id();
-        if (i < kMlpParams) s_mlp[mlp_t_index(i)] = wreg[k];
-      }
+      __builtin_amdgcn_s_waitcnt(0);  // the weights are in LDS
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < 12; ++k) f[k] = __shfl(f[k], 0);
@@ -1187,6 +1187,8 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       if (phase == kBDecide) ctl->t_decide = (int64_t)wall_clock64();
     }
   }
+  // no copy into LDS may still be in flight when the wave ends
+  if (mlp_on) __builtin_amdgcn_s_waitcnt(0);
   if (threadIdx.x == 0) __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -1416,7 +1418,7 @@ static BatchParams<OffT> batch_params(fc_graph* g, BatchWorkspace* b) {
   p.qt = b->qt;
   p.dist = b->dist;
   p.perm = g->relabeled ? g->d_perm : nullptr;
-  p.mlp = g->d_mlp;
+  p.mlp = g->d_mlp + kMlpParams;  // the copy in mlp_t_index order
   return p;
 }
 

@@ -314,8 +314,9 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
     }
   }
   g->d_ctl = (fcd::VisitCtl*)(g->d_ws + off_ctl);
-  CK(hipMalloc((void**)&g->d_mlp, sizeof(float) * fcd::kMlpParams));
-  CK(hipMemset(g->d_mlp, 0, sizeof(float) * fcd::kMlpParams));
+  // canonical parameters, then the copy in the batch kernel's layout
+  CK(hipMalloc((void**)&g->d_mlp, 2 * sizeof(float) * fcd::kMlpParams));
+  CK(hipMemset(g->d_mlp, 0, 2 * sizeof(float) * fcd::kMlpParams));
   CK(hipHostMalloc((void**)&g->h_phase, 4 * sizeof(int32_t), hipHostMallocDefault));
 
   int occ = sizeof(OffT) == 8 ? fcd::phase_occupancy<int64_t>() : fcd::phase_occupancy<int32_t>();

@@ -69,7 +69,7 @@ struct fc_graph {
 
   bool parents_valid = false;
   bool levels_valid = false;
-  float* d_mlp = nullptr;           // scheduler parameters (device)
+  float* d_mlp = nullptr;           // scheduler parameters (device): canonical, then in mlp_t_index order
   bool mlp_set = false;
   std::vector<float> h_mlp;
 

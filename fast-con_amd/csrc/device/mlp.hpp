@@ -12,7 +12,7 @@ namespace fcd {
 // Position of canonical parameter i in the wave-friendly layout used by
 // mlp_pick_wave_t: W1, W2 and W3 transposed (input-major), so that the lanes
 // of one step read consecutive words; biases unchanged.
-__device__ __forceinline__ int mlp_t_index(int i) {
+__host__ __device__ inline int mlp_t_index(int i) {
   constexpr int oB1 = kHidden * kFeatures, oW2 = oB1 + kHidden, oB2 = oW2 + kHidden * kHidden;
   constexpr int oW3 = oB2 + kHidden, oB3 = oW3 + kActions * kHidden;
   if (i < oB1) return (i % kFeatures) * kHidden + i / kFeatures;

@@ -220,7 +220,11 @@ static void init_params(uint64_t seed, std::vector<float>& w) {
 
 static fc_status set_params_device(fc_graph* g, const std::vector<float>& w) {
   g->h_mlp = w;
+  // the batch kernel reads a second copy in its wave-friendly layout
+  std::vector<float> wt((size_t)kP);
+  for (int i = 0; i < kP; ++i) wt[(size_t)mlp_t_index(i)] = w[(size_t)i];
   FC_HIP(hipMemcpyAsync(g->d_mlp, g->h_mlp.data(), sizeof(float) * (size_t)kP, hipMemcpyHostToDevice, g->stream));
+  FC_HIP(hipMemcpyAsync(g->d_mlp + kP, wt.data(), sizeof(float) * (size_t)kP, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipStreamSynchronize(g->stream));
   g->mlp_set = true;
   return FC_OK;
