@@ -2,10 +2,18 @@
 // fastcon, one GPU.
 //
 //   st-con -S SCALE [-E edgefactor] [-1 seed1] [-2 seed2] [-N pairs] [-G 0|1]
-//          [-f edge_list_file] [-d stats_dir] [-R red -B blue] [-v]
-//          (-t -b -D -T -V -s -r -m are accepted for compatibility; they set
-//           CUDA launch geometry, MPI buffers and validation depth, which
-//           this engine does not have)
+//          [-f edge_list_file] [-d stats_dir] [-R red -B blue] [-V num] [-v]
+//          (-t -b -D -T -s -r -m are accepted for compatibility; they set
+//           CUDA launch geometry and MPI buffers, which this engine does not
+//           have)
+//
+// -V num (default 1, as the reference; 0 turns it off) validates up to num
+// matching nodes per connected pair with validate_stcon_bfs's rule
+// (st-conn/make_stcon.c:516-575,1578-1641): a BFS from the matching node,
+// whose tree parents from t and from s must reach it within DEPTH steps,
+// the longer walk taking exactly DEPTH. This engine reports one matching
+// node per pair. A failed check prints the reference's ERROR line and stops
+// the run with exit status 2.
 //
 // For each pair it prints the reference's START / ST-CON RUN / END lines and
 // writes an st_stats_<date>_<time> file in the reference's format
@@ -67,20 +75,21 @@ Stats statistics(const std::vector<double>& x) {
 void usage(const char* prog) {
   std::fprintf(stderr,
                "Usage: %s -S SCALE -E Edgefactor -1 seed1 -2 seed2 -N num_pairs [-G 0|1] [-f filename]"
-               " [-d stats_dir] [-R vertex -B vertex] [-v]\n"
+               " [-d stats_dir] [-R vertex -B vertex] [-V num] [-v]\n"
                "SCALE = log_2(# vertices) [integer, required unless -f]\n"
                "  edgefactor = (# edges) / (# vertices) = .5 * (average vertex degree) [float, defaults to 16]\n"
                "-f filename to run on the real graph stored in filename (edge list)\n"
                "-d directory to store the st_stats file\n"
                "-G (0|1) 0 for RMAT graph and 1 for Random graph\n"
-               "-R vertex -B vertex (red and blue vertices)\n",
+               "-R vertex -B vertex (red and blue vertices)\n"
+               "-V num (Maximum number of matching nodes to validate, default 1)\n",
                prog);
 }
 
 }  // namespace
 
 int main(int argc, char* argv[]) {
-  int scale = 0, num_pairs = 64, gen_type = 0, errflg = 0;
+  int scale = 0, num_pairs = 64, gen_type = 0, errflg = 0, validation = 1;
   double edgefactor = 16.0;
   uint64_t seed1 = 1, seed2 = 2;
   long long red = -1, blue = -1;
@@ -106,7 +115,8 @@ int main(int argc, char* argv[]) {
           errflg++;
         }
         break;
-      case 't': case 'b': case 'D': case 'T': case 'V': break;
+      case 'V': validation = std::atoi(optarg); break;
+      case 't': case 'b': case 'D': case 'T': break;
       case 'v': std::printf("Version %s\n", fc_version()); return 0;
       default: errflg++; break;
     }
@@ -200,6 +210,20 @@ int main(int argc, char* argv[]) {
     }
     std::fprintf(fs, "%d %d %d %d %d %g %g\n", i, s[i], t[i], depth, conn ? 1 : 0, sec, teps);
     std::fprintf(stdout, "END STCON [%d]\n", i);
+    if (conn && validation > 0 && s[i] != t[i]) {
+      const fc_status vs = fc_validate_stcon(g, s[i], t[i], meet, depth);
+      if (vs == FC_OK) {
+        std::fprintf(stdout, "ST-CON Validation %d OK with Matching node %d\n", 0, meet);
+      } else {
+        std::fprintf(stderr, "%s\n", fc_last_error());
+        std::fprintf(stdout, "ST-CON Validation %d ERROR with Matching node %d\n", 0, meet);
+        std::fprintf(stdout, "seed1=%" PRIu64 ", seed2=%" PRIu64 "\n", seed1, seed2);
+        std::fclose(fs);
+        fc_graph_destroy(g);
+        fc_host_graph_free(&hg);
+        return 2;
+      }
+    }
     times.push_back(sec);
     conps.push_back(sec > 0 ? 1.0 / sec : 0.0);
     visited.push_back((double)st.visited_vertices);

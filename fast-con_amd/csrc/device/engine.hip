@@ -522,6 +522,173 @@ extern "C" fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t*
   return stcon_run(g, s, t, n, fcd::kModePlainSt, 0, connected, dist, nullptr, stats);
 }
 
+// Shortest s–t path from the parents a plain visit recorded (one wave).
+// The meeting vertex u has level lu in its own colour's tree; the other end
+// of the meeting edge is a neighbour w of u of the other colour at level
+// dist - 1 - lu. The lanes search u's adjacency for w; lane 0 then walks
+// both parent chains (a root is its own parent) and writes the path, s
+// first, into out[0..dist]. *status receives dist + 1, or a negative code:
+// -1 no such neighbour, -2 a chain left the vertex range, -3 a chain did not
+// end at a root.
+template <typename OffT>
+__global__ void path_kernel(const OffT* __restrict__ off, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ state, const int32_t* __restrict__ parent, int32_t V,
+                            int32_t u, int32_t dist, int32_t* __restrict__ out, int32_t* __restrict__ status) {
+  using namespace fcd;
+  const int lane = lane_id();
+  const int32_t su = state[u];
+  const bool tu = (su & kColorBit) != 0;  // u on the t side
+  const int32_t lu = su & kLevelMask;
+  const int32_t want = dist - 1 - lu;
+  int32_t w = -1;
+  const OffT e1 = off[u + 1];
+  for (OffT b = off[u]; b < e1; b += kWave) {
+    const OffT e = b + (OffT)lane;
+    int32_t x = -1;
+    bool hit = false;
+    if (e < e1) {
+      x = col[e];
+      const int32_t sx = state[x];
+      hit = sx != -1 && (((sx & kColorBit) != 0) != tu) && (sx & kLevelMask) == want;
+    }
+    const uint64_t m = ballot(hit);
+    if (m) {
+      w = __shfl(x, __ffsll((unsigned long long)m) - 1);
+      break;
+    }
+  }
+  if (lane != 0) return;
+  if (w < 0 || want < 0) {
+    *status = -1;
+    return;
+  }
+  const int32_t a = tu ? w : u, b = tu ? u : w;  // s-side and t-side ends
+  const int32_t la = tu ? want : lu;             // level of a from s
+  int32_t x = a;
+  for (int32_t k = la; k >= 0; --k) {
+    if ((uint32_t)x >= (uint32_t)V) {
+      *status = -2;
+      return;
+    }
+    out[k] = x;
+    x = parent[x];
+  }
+  if (x != out[0]) {  // out[0] must be the s root, its own parent
+    *status = -3;
+    return;
+  }
+  x = b;
+  for (int32_t k = la + 1; k <= dist; ++k) {
+    if ((uint32_t)x >= (uint32_t)V) {
+      *status = -2;
+      return;
+    }
+    out[k] = x;
+    x = parent[x];
+  }
+  if (x != out[dist]) {  // out[dist] must be the t root
+    *status = -3;
+    return;
+  }
+  *status = dist + 1;
+}
+
+extern "C" fc_status fc_stcon_path(fc_graph* g, int32_t s, int32_t t, int32_t* path_out, int32_t cap, int32_t* len_out,
+                                   fc_stats* stats) {
+  fc::clear_error();
+  if (!g || !len_out || (cap > 0 && !path_out)) return fail(FC_ERR_INVALID, "fc_stcon_path: null argument");
+  if (s < 0 || s >= g->V || t < 0 || t >= g->V)
+    return fail(FC_ERR_INVALID, "fc_stcon_path: vertex out of range [0,%lld)", (long long)g->V);
+  g->levels_valid = g->parents_valid = false;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  *len_out = 0;
+  if (s == t) {
+    *len_out = 1;
+    if (cap < 1) return fail(FC_ERR_CAPACITY, "fc_stcon_path: the path needs 1 entry, cap is %d", cap);
+    path_out[0] = s;
+    return FC_OK;
+  }
+  const int32_t ds = g->relabeled ? g->h_perm[(size_t)s] : s;
+  const int32_t dt = g->relabeled ? g->h_perm[(size_t)t] : t;
+  double ms = 0;
+  fc_status st = fcd::run_visits(g, fcd::kModePlainSt, 0, &ds, &dt, 1, true, &ms);
+  if (st != FC_OK) return st;
+  const fcd::VisitOut o = g->h_out[0];
+  st = check_out(o);
+  if (st != FC_OK) return st;
+  if (stats) {
+    stats->time_ms = ms;
+    add_stats(o, stats);
+  }
+  if (o.dist < 0) return FC_OK;
+  *len_out = o.dist + 1;
+  if (o.dist + 1 > cap)
+    return fail(FC_ERR_CAPACITY, "fc_stcon_path: the path needs %d entries, cap is %d", o.dist + 1, cap);
+  if ((uint32_t)o.meet >= (uint32_t)g->V || o.dist + 1 > g->V)
+    return fail(FC_ERR_CAPACITY, "fc_stcon_path: visit returned meet %d at distance %d", o.meet, o.dist);
+  // The frontier queues are idle after the visit: the path goes to the
+  // first, the status word to the second.
+  int32_t* d_path = g->d_q[0][0];
+  int32_t* d_status = g->d_q[1][0];
+  FC_HIP(hipEventRecord(g->ev0, g->stream));
+  if (g->off64)
+    hipLaunchKernelGGL(path_kernel<int64_t>, dim3(1), dim3(fcd::kWave), 0, g->stream, (const int64_t*)g->d_off,
+                       g->d_col, g->d_state, g->d_parent, (int32_t)g->V, o.meet, o.dist, d_path, d_status);
+  else
+    hipLaunchKernelGGL(path_kernel<int32_t>, dim3(1), dim3(fcd::kWave), 0, g->stream, (const int32_t*)g->d_off,
+                       g->d_col, g->d_state, g->d_parent, (int32_t)g->V, o.meet, o.dist, d_path, d_status);
+  FC_HIP(hipGetLastError());
+  FC_HIP(hipEventRecord(g->ev1, g->stream));
+  int32_t status = 0;
+  FC_HIP(hipMemcpyAsync(&status, d_status, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+  FC_HIP(hipMemcpyAsync(path_out, d_path, sizeof(int32_t) * (size_t)(o.dist + 1), hipMemcpyDeviceToHost, g->stream));
+  FC_HIP(hipStreamSynchronize(g->stream));
+  float pms = 0.f;
+  FC_HIP(hipEventElapsedTime(&pms, g->ev0, g->ev1));
+  if (stats) stats->time_ms += pms;
+  if (status != o.dist + 1) return fail(FC_ERR_HIP, "fc_stcon_path: path assembly failed (code %d)", status);
+  if (g->relabeled)
+    for (int32_t k = 0; k <= o.dist; ++k) path_out[k] = g->h_iperm[(size_t)path_out[k]];
+  if (path_out[0] != s || path_out[o.dist] != t)
+    return fail(FC_ERR_HIP, "fc_stcon_path: assembled path runs %d..%d", path_out[0], path_out[o.dist]);
+  return FC_OK;
+}
+
+// validate_stcon_bfs (st-conn/make_stcon.c:1578-1641) on one GPU: the BFS
+// from mn runs on the device, the parent walks on the host.
+extern "C" fc_status fc_validate_stcon(fc_graph* g, int32_t s, int32_t t, int32_t mn, int32_t depth) {
+  fc::clear_error();
+  if (!g) return fail(FC_ERR_INVALID, "fc_validate_stcon: null graph");
+  if (s < 0 || s >= g->V || t < 0 || t >= g->V || mn < 0 || mn >= g->V)
+    return fail(FC_ERR_INVALID, "fc_validate_stcon: vertex out of range [0,%lld)", (long long)g->V);
+  std::vector<int32_t> lv((size_t)g->V), par((size_t)g->V);
+  fc_status st = fc_bfs(g, mn, lv.data(), par.data(), nullptr);
+  if (st != FC_OK) return st;
+  int32_t dep[2] = {-1, -1};
+  const int32_t from[2] = {t, s};  // blue path first, as the reference
+  for (int k = 0; k < 2; ++k) {
+    int32_t v = from[k], p = v;
+    int64_t steps = -1;
+    do {
+      v = p;
+      if (v < 0 || v >= g->V) return fail(FC_ERR_MISMATCH, "%s path leaves the graph", k ? "RED" : "BLUE");
+      p = par[(size_t)v];
+      ++steps;
+    } while (v != p && steps <= g->V);
+    if (v != mn)
+      return fail(FC_ERR_MISMATCH, "WRONG MATCHING NODE: %s Path leaf = %d, Matching Node = %d", k ? "RED" : "BLUE", v,
+                  mn);
+    dep[k] = (int32_t)steps;
+    if (steps > depth)
+      return fail(FC_ERR_MISMATCH, "WRONG ST-CON depth on %s: ST-CON depth = %d, bfs depth = %d", k ? "RED" : "BLUE",
+                  depth, dep[k]);
+  }
+  const int32_t mx = std::max(dep[0], dep[1]);
+  if (mx != depth)
+    return fail(FC_ERR_MISMATCH, "WRONG ST-CON MAX depth: ST-CON MAX depth = %d, bfs MAX depth = %d", depth, mx);
+  return FC_OK;
+}
+
 extern "C" fc_status fc_stcon_sched_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy,
                                          int32_t* connected, int32_t* dist, fc_stats* stats) {
   if (policy == FC_POLICY_PLAIN) return fc_stcon_many(g, s, t, n, connected, dist, stats);

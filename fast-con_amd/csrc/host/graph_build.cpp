@@ -197,6 +197,92 @@ extern "C" fc_status fc_sample_roots(const fc_host_graph* g, int32_t n, uint64_t
   return FC_OK;
 }
 
+// Graph500 BFS-tree validation (validate_bfs_result,
+// st-conn/reference_functions.c:99-434) for one rank. The depths are derived
+// from the parents alone, as the reference's mini-BFS over the predecessor
+// map does: here each chain is walked up to a vertex of known depth and the
+// depths are filled in on the way back, so the pass is linear; a chain
+// longer than V or one that ends outside the root's tree is not a tree.
+extern "C" fc_status fc_validate_bfs(const fc_host_graph* g, int32_t root, const int32_t* parents, int64_t nvisited,
+                                     int32_t* depth_out) {
+  fc::clear_error();
+  if (!g || !parents || (g->V > 0 && (!g->offsets || (g->E > 0 && !g->columns))))
+    return fail(FC_ERR_INVALID, "fc_validate_bfs: null argument");
+  const int64_t V = g->V;
+  if (root < 0 || root >= V) return fail(FC_ERR_INVALID, "fc_validate_bfs: root %d out of range", root);
+  if (parents[root] != root)
+    return fail(FC_ERR_MISMATCH, "Validation error: parent of root vertex %d is %d, not the root itself.", root,
+                parents[root]);
+  int64_t visited = 0;
+  for (int64_t v = 0; v < V; ++v) {
+    const int32_t p = parents[v];
+    if (v != root && p == v)
+      return fail(FC_ERR_MISMATCH, "Validation error: parent of non-root vertex %lld is itself.", (long long)v);
+    if (p < -1 || p >= V)
+      return fail(FC_ERR_MISMATCH, "Validation error: parent of vertex %lld is out-of-range value %d.", (long long)v, p);
+    visited += p != -1;
+  }
+  if (nvisited >= 0 && visited != nvisited)
+    return fail(FC_ERR_MISMATCH, "Validation error: claimed visit count %lld is different from actual count %lld.",
+                (long long)nvisited, (long long)visited);
+  int64_t bad = -1;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : bad)
+  for (int64_t v = 0; v < V; ++v) {
+    const int32_t p = parents[v];
+    if (p == -1 || p == v) continue;
+    bool found = false;
+    for (int64_t e = g->offsets[v]; e < g->offsets[v + 1] && !found; ++e) found = g->columns[e] == p;
+    if (!found) bad = std::max(bad, v);
+  }
+  if (bad >= 0)
+    return fail(FC_ERR_MISMATCH, "Validation error: no graph edge from vertex %lld to its parent %d.", (long long)bad,
+                parents[bad]);
+  // depths from the parents; -2 = not yet known, -1 = outside the tree
+  std::vector<int32_t> depth((size_t)V, -2);
+  std::vector<int32_t> chain;
+  depth[(size_t)root] = 0;
+  for (int64_t v = 0; v < V; ++v) {
+    if (depth[(size_t)v] != -2) continue;
+    if (parents[v] == -1) {
+      depth[(size_t)v] = -1;
+      continue;
+    }
+    chain.clear();
+    int64_t x = v;
+    while (depth[(size_t)x] == -2 && parents[x] != -1 && (int64_t)chain.size() <= V) {
+      chain.push_back((int32_t)x);
+      x = parents[x];
+    }
+    const int32_t dx = depth[(size_t)x];
+    if (dx < 0 || (int64_t)chain.size() > V)
+      return fail(FC_ERR_MISMATCH,
+                  "Validation error: BFS predecessors do not form a tree; see vertex %lld and its parent %d.",
+                  (long long)v, parents[v]);
+    for (size_t k = chain.size(); k-- > 0;) depth[(size_t)chain[k]] = dx + (int32_t)(chain.size() - k);
+  }
+  int64_t far = -1, out = -1;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : far, out)
+  for (int64_t v = 0; v < V; ++v) {
+    const int32_t dv = depth[(size_t)v];
+    for (int64_t e = g->offsets[v]; e < g->offsets[v + 1]; ++e) {
+      const int32_t dw = depth[(size_t)g->columns[e]];
+      if ((dv < 0) != (dw < 0))
+        out = std::max(out, v);
+      else if (dv >= 0 && (dv - dw > 1 || dw - dv > 1))
+        far = std::max(far, v);
+    }
+  }
+  if (out >= 0)
+    return fail(FC_ERR_MISMATCH, "Validation error: edge connects vertex %lld to a vertex on the other side of the BFS tree "
+                "boundary (depth %d).", (long long)out, depth[(size_t)out]);
+  if (far >= 0)
+    return fail(FC_ERR_MISMATCH,
+                "Validation error: depths of edge endpoints are too far apart (abs. val. > 1) at vertex %lld (depth %d).",
+                (long long)far, depth[(size_t)far]);
+  if (depth_out) std::copy(depth.begin(), depth.end(), depth_out);
+  return FC_OK;
+}
+
 namespace fc {
 
 // Structural check of a caller-supplied CSR before it is uploaded: offsets

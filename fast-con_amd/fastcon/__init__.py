@@ -27,7 +27,8 @@ if not os.path.exists(LIB_PATH):
 _lib = C.CDLL(LIB_PATH)
 
 FC_INF = -1
-OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_IO, ERR_FORMAT, ERR_CAPACITY, ERR_NOGPU, ERR_TIMEOUT, ERR_UNSUPPORTED = range(10)
+(OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_IO, ERR_FORMAT, ERR_CAPACITY, ERR_NOGPU, ERR_TIMEOUT, ERR_UNSUPPORTED,
+ ERR_MISMATCH) = range(11)
 POLICY_PLAIN, POLICY_HEURISTIC, POLICY_LEARNED = 0, 1, 2
 GRAPH_KEEP_ORDER = 1
 GRAPH_KEEP_IDS = 2
@@ -110,6 +111,9 @@ _sig("fc_graph_levels", C.c_int, C.c_void_p, _i32p)
 _sig("fc_graph_parents", C.c_int, C.c_void_p, _i32p)
 _sig("fc_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.POINTER(Stats))
 _sig("fc_stcon_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(Stats))
+_sig("fc_stcon_path", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int32, _i32p, C.POINTER(Stats))
+_sig("fc_validate_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32)
+_sig("fc_validate_bfs", C.c_int, C.POINTER(_HostGraph), C.c_int32, _i32p, C.c_int64, _i32p)
 _sig("fc_stcon_sched_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
      C.POINTER(Stats))
 _sig("fc_stcon_batch", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(BatchOpts),
@@ -130,7 +134,7 @@ EXPORTED = [
     "fc_graph_create_csr32", "fc_graph_create_csr64", "fc_graph_destroy", "fc_bfs", "fc_graph_levels",
     "fc_graph_parents", "fc_stcon", "fc_stcon_many", "fc_stcon_sched_many", "fc_stcon_batch",
     "fc_sched_trace", "fc_sched_train", "fc_sched_set_params", "fc_sched_get_params", "fc_sched_forward",
-    "fc_sched_gradient",
+    "fc_sched_gradient", "fc_stcon_path", "fc_validate_stcon", "fc_validate_bfs",
 ]
 
 
@@ -197,6 +201,21 @@ class HostGraph:
         hg = self._struct()
         _check(_lib.fc_components(C.byref(hg), _p32(out)))
         return out
+
+    def validate_bfs(self, root: int, parents, nvisited: int = -1):
+        """Graph500 BFS-tree check (fc_validate_bfs). Returns (ok, message,
+        depths); depths is None unless ok."""
+        parents = np.ascontiguousarray(parents, np.int32)
+        if len(parents) != self.V:
+            raise ValueError("parents must have V entries")
+        depth = np.empty(self.V, np.int32)
+        hg = self._struct()
+        st = _lib.fc_validate_bfs(C.byref(hg), int(root), _p32(parents), int(nvisited), _p32(depth))
+        if st == OK:
+            return True, "", depth
+        if st == ERR_MISMATCH:
+            return False, _lib.fc_last_error().decode(errors="replace"), None
+        _check(st)
 
 
 def gen_rmat(scale: int, edgefactor: float = 16.0, a: float = 0.57, b: float = 0.19, c: float = 0.19,
@@ -308,6 +327,30 @@ class Graph:
         st = Stats()
         _check(_lib.fc_stcon(self._h, s, t, C.byref(conn), C.byref(dist), C.byref(meet), C.byref(st)))
         return bool(conn.value), dist.value, meet.value, st.as_dict()
+
+    def stcon_path(self, s: int, t: int, cap: int = 4096):
+        """A shortest s–t path (fc_stcon_path): (path as int32 array, s
+        first; empty when not connected), stats."""
+        for _ in range(2):
+            out = np.empty(max(cap, 1), np.int32)
+            n = C.c_int32()
+            st = Stats()
+            code = _lib.fc_stcon_path(self._h, int(s), int(t), _p32(out), cap, C.byref(n), C.byref(st))
+            if code == ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            _check(code)
+            return out[: n.value].copy(), st.as_dict()
+        _check(code)
+
+    def validate_stcon(self, s: int, t: int, mn: int, depth: int):
+        """ST-CON's matching-node check (fc_validate_stcon): (ok, message)."""
+        code = _lib.fc_validate_stcon(self._h, int(s), int(t), int(mn), int(depth))
+        if code == OK:
+            return True, ""
+        if code == ERR_MISMATCH:
+            return False, _lib.fc_last_error().decode(errors="replace")
+        _check(code)
 
     def stcon_many(self, s, t, policy: int = POLICY_PLAIN) -> StResult:
         s = np.ascontiguousarray(s, np.int32)

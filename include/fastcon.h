@@ -57,7 +57,8 @@ typedef enum fc_status {
   FC_ERR_CAPACITY = 6,  /* a fixed-capacity buffer would overflow (BFS-4K/src/device/BFS_WorkEfficient.cu:111-112) */
   FC_ERR_NOGPU = 7,     /* no HIP device, or the device code object is missing */
   FC_ERR_TIMEOUT = 8,   /* a device-side bounded wait expired; results are invalid */
-  FC_ERR_UNSUPPORTED = 9
+  FC_ERR_UNSUPPORTED = 9,
+  FC_ERR_MISMATCH = 10  /* a validator found a wrong result (validate_bfs_result, validate_stcon_bfs) */
 } fc_status;
 
 /* Text of the last error raised on the calling thread ("" if none). */
@@ -130,6 +131,18 @@ fc_status fc_sample_roots(const fc_host_graph* g, int32_t n, uint64_t seed, int3
 /* Connected-component labels (label = smallest vertex id of the component). */
 fc_status fc_components(const fc_host_graph* g, int32_t* label_out);
 
+/* Graph500 BFS-tree validation on the host graph (validate_bfs_result,
+ * st-conn/reference_functions.c:99-434), one rank: the root is its own
+ * parent and no other vertex is; parents are -1 or vertex ids; nvisited
+ * (ignored when < 0) equals the number of vertices with a parent; each
+ * parent is a neighbour; the parents form a tree (depths follow from the
+ * root); every edge joins two tree vertices whose depths differ by at most
+ * one, or two vertices outside the tree. depth_out (nullable, V entries)
+ * receives the tree depths, -1 outside the tree. FC_OK, or FC_ERR_MISMATCH
+ * with the first violation in the reference's wording. */
+fc_status fc_validate_bfs(const fc_host_graph* g, int32_t root, const int32_t* parents, int64_t nvisited,
+                          int32_t* depth_out);
+
 /* ------------------------------------------------------------------------ */
 /* Device graphs and queries.                                                */
 /* ------------------------------------------------------------------------ */
@@ -193,6 +206,26 @@ fc_status fc_stcon(fc_graph* g, int32_t s, int32_t t, int32_t* connected, int32_
  * st-conn/make_stcon.c:418-452). */
 fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
                         int32_t* connected, int32_t* dist, fc_stats* stats);
+
+/* One plain query with a shortest path, in caller ids: path_out[0] = s,
+ * path_out[dist] = t and *len_out = dist + 1 (0 when not connected, 1 when
+ * s == t). cap must be at least dist + 1, else FC_ERR_CAPACITY with
+ * *len_out set to the length needed. The visit records each claimed
+ * vertex's parent; the path joins the s-side and t-side parent chains at the
+ * meeting edge. Replaces the predecessor reconstruction of make_stcon
+ * (st-conn/make_stcon.c:1209-1455; stcon_remove_pred, stcon_pred_local_host,
+ * st-conn/make_stcon_func.cu:238-392). stats as fc_stcon, time including the
+ * path assembly on the device. */
+fc_status fc_stcon_path(fc_graph* g, int32_t s, int32_t t, int32_t* path_out, int32_t cap,
+                        int32_t* len_out, fc_stats* stats);
+
+/* ST-CON's check of a matching node mn (validate_stcon_bfs,
+ * st-conn/make_stcon.c:1578-1641): a BFS rooted at mn; following its tree
+ * parents from t and from s must reach mn within depth steps each, and the
+ * longer walk must take exactly depth steps (depth = ST-CON's DEPTH,
+ * ceil(dist/2)). FC_OK when it holds, FC_ERR_MISMATCH with the reason
+ * otherwise. */
+fc_status fc_validate_stcon(fc_graph* g, int32_t s, int32_t t, int32_t mn, int32_t depth);
 
 /* Plain, heuristic or learned scheduling of one-query-per-launch visits
  * (policy: fc_policy). FC_POLICY_PLAIN is fc_stcon_many. The heuristic and

@@ -100,6 +100,12 @@ def test_stcon_cli_stats_file(fc, orc, tmp_path):
             assert depth == dp == (d + 1) // 2
     assert text[head + 13].startswith("max-ST-CON-level ")
     assert r.stdout.count("END STCON [") == 12
+    # validation is on by default, as the reference's -V 1: one OK per
+    # connected pair with s != t
+    nconn = sum(1 for row in rows if int(row[4]) and row[1] != row[2])
+    assert r.stdout.count("ST-CON Validation 0 OK with Matching node") == nconn and "ERROR" not in r.stdout
+    r = _run([os.path.join(BIN, "st-con"), "-f", path, "-N", "4", "-2", "9", "-V", "0", "-d", str(tmp_path)])
+    assert r.returncode == 0 and "ST-CON Validation" not in r.stdout
     r = _run([os.path.join(BIN, "st-con"), "-v"])
     assert r.returncode == 0 and r.stdout.startswith("Version ")
     r = _run([os.path.join(BIN, "st-con")])

@@ -213,3 +213,70 @@ def test_layout_does_not_change_answers(fc, orc, keep_order, keep_ids):
                 ds, _, _ = orc.bfs(hg.offsets, hg.columns, int(s[k]))
                 dt, _, _ = orc.bfs(hg.offsets, hg.columns, int(t[k]))
                 assert ds[meet] >= 0 and dt[meet] >= 0 and ds[meet] + dt[meet] == d
+
+
+def _is_path(hg, path):
+    off, col = hg.offsets, hg.columns
+    return all(int(y) in set(col[off[x]:off[x + 1]].tolist()) for x, y in zip(path[:-1], path[1:]))
+
+
+@pytest.mark.parametrize("keep_ids", [False, True])
+def test_stcon_path_is_a_shortest_path(fc, orc, keep_ids):
+    """fc_stcon_path returns s ... t of exactly dist + 1 vertices, each step an
+    edge, on R-MAT and on a grid (long paths); empty when unconnected, [s]
+    when s == t; a too-small cap reports the length needed."""
+    hg = fc.gen_rmat(14, 16, seed=31)
+    roots = hg.sample_roots(64, 32)
+    grid = fc.gen_grid(64, 64, 0.05, 3)
+    groots = grid.sample_roots(16, 4)
+    for name, h, s, t in (("rmat14", hg, roots[:32], roots[32:]), ("grid64", grid, groots[:8], groots[8:])):
+        say(f"path {name} keep_ids={keep_ids}")
+        with fc.Graph(h, keep_ids=keep_ids) as g:
+            for a, b in zip(s, t):
+                conn, d, _ = orc.stcon(h.offsets, h.columns, int(a), int(b))
+                path, st = g.stcon_path(int(a), int(b))
+                if not conn:
+                    assert len(path) == 0
+                    continue
+                assert len(path) == d + 1 and path[0] == a and path[-1] == b, (a, b, d, path)
+                assert _is_path(h, path)
+            p0, _ = g.stcon_path(int(s[0]), int(s[0]))
+            assert p0.tolist() == [int(s[0])]
+            conn, d, _ = orc.stcon(h.offsets, h.columns, int(s[1]), int(t[1]))
+            if conn and d >= 2:
+                out = np.empty(2, np.int32)
+                n = fc.C.c_int32()
+                st = fc.Stats()
+                code = fc.lib().fc_stcon_path(g._h, int(s[1]), int(t[1]), out.ctypes.data_as(fc._i32p), 1,
+                                              fc.C.byref(n), fc.C.byref(st))
+                assert code == fc.ERR_CAPACITY and n.value == d + 1
+    # two components: no path
+    two = fc.csr_from_edges(np.array([[0, 1], [1, 2], [3, 4]]), 5)
+    with fc.Graph(two, keep_ids=keep_ids) as g:
+        path, _ = g.stcon_path(0, 4)
+        assert len(path) == 0
+        path, _ = g.stcon_path(2, 0)
+        assert path.tolist() == [2, 1, 0]
+
+
+def test_validate_stcon_matching_nodes(fc, orc):
+    """fc_validate_stcon (validate_stcon_bfs) accepts the engine's meeting
+    vertices with DEPTH = ceil(d/2), and rejects a wrong depth and a vertex
+    off the meeting level."""
+    hg = fc.gen_rmat(13, 16, seed=41)
+    roots = hg.sample_roots(32, 42)
+    with fc.Graph(hg) as g:
+        checked = 0
+        for a, b in zip(roots[:16], roots[16:]):
+            conn, d, meet, _ = g.stcon(int(a), int(b))
+            if not conn or d < 2:
+                continue
+            depth = (d + 1) // 2
+            ok, msg = g.validate_stcon(int(a), int(b), meet, depth)
+            assert ok, msg
+            ok, msg = g.validate_stcon(int(a), int(b), meet, depth + 1)
+            assert not ok and "MAX depth" in msg
+            ok, msg = g.validate_stcon(int(a), int(b), int(a), depth)
+            assert not ok and "depth on BLUE" in msg
+            checked += 1
+        assert checked >= 8

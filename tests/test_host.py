@@ -17,7 +17,7 @@ def test_library_exports_every_declared_symbol(fc):
     lib = fc.lib()
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
-    assert set(fc.EXPORTED) >= declared - {"fc_free"} or True
+    assert declared <= set(fc.EXPORTED), sorted(declared - set(fc.EXPORTED))
     assert fc.version().startswith("fastcon")
 
 
@@ -184,3 +184,65 @@ def test_read_edge_list(fc, tmp_path):
     txt = "# Directed graph\n# Nodes: 6 Edges: 3\n# FromNodeId\tToNodeId\n0 5\n3 2\n5 5\n"
     e, nv = fc.read_edge_list(_write(tmp_path, "e.txt", txt))
     assert nv == 6 and e.tolist() == [[0, 5], [3, 2], [5, 5]]
+
+
+def _tree_case(fc, orc, hg, root):
+    """Oracle levels and a BFS tree built from them (each reached vertex's
+    smallest neighbour one level up)."""
+    lv, _, _ = orc.bfs(hg.offsets, hg.columns, root)
+    pa = np.full(hg.V, -1, np.int32)
+    pa[root] = root
+    off, col = hg.offsets, hg.columns
+    for v in np.flatnonzero(lv > 0):
+        nb = col[off[v]:off[v + 1]]
+        pa[v] = nb[lv[nb] == lv[v] - 1].min()
+    return lv, pa
+
+
+def test_validate_bfs_accepts_oracle_trees_and_names_planted_mistakes(fc, orc):
+    """fc_validate_bfs (Graph500 validate_bfs_result) accepts the oracle's
+    BFS trees, returns their depths, and rejects each planted mistake with
+    the reference's message."""
+    hg = fc.gen_rmat(12, 8, seed=3)
+    roots = hg.sample_roots(3, 4)
+    for r in roots:
+        lv, pa = _tree_case(fc, orc, hg, int(r))
+        ok, msg, depth = hg.validate_bfs(int(r), pa, nvisited=int((pa != -1).sum()))
+        assert ok, msg
+        assert np.array_equal(depth, lv)
+        assert orc.validate_bfs_tree(hg.offsets, hg.columns, int(r), lv, pa) == 0
+    r = int(roots[0])
+    lv, pa = _tree_case(fc, orc, hg, r)
+    reached = np.flatnonzero((lv >= 2))
+    v = int(reached[0])
+    cases = []
+    bad = pa.copy(); bad[r] = int(pa[v]); cases.append(("root", bad, "parent of root vertex"))
+    bad = pa.copy(); bad[v] = v; cases.append(("self", bad, "is itself"))
+    bad = pa.copy(); bad[v] = hg.V + 5; cases.append(("range", bad, "out-of-range"))
+    # a parent that is not a neighbour
+    nb = set(hg.columns[hg.offsets[v]:hg.offsets[v + 1]].tolist())
+    far = next(int(x) for x in np.flatnonzero(lv >= 0) if int(x) not in nb and int(x) != v)
+    bad = pa.copy(); bad[v] = far; cases.append(("edge", bad, "no graph edge"))
+    # a two-cycle between neighbours away from the root
+    u = next(int(x) for x in hg.columns[hg.offsets[v]:hg.offsets[v + 1]] if int(x) != r and lv[int(x)] >= 1)
+    bad = pa.copy(); bad[v] = u; bad[u] = v; cases.append(("cycle", bad, "do not form a tree"))
+    # a leaf of the tree marked unreached, although its neighbours are reached
+    leaf = next(int(x) for x in reached if not (pa == x).any())
+    bad = pa.copy(); bad[leaf] = -1; cases.append(("outside", bad, "BFS tree"))
+    for name, b, text in cases:
+        ok, msg, _ = hg.validate_bfs(r, b)
+        assert not ok and text in msg, (name, msg)
+    ok, msg, _ = hg.validate_bfs(r, pa, nvisited=int((pa != -1).sum()) + 1)
+    assert not ok and "visit count" in msg
+
+
+def test_validate_bfs_depth_gap(fc, orc):
+    """A parent tree that is a valid tree but not a BFS tree (an edge whose
+    ends differ in depth by two) is rejected."""
+    # path 0-1-2-3 plus the chord 0-2: the tree 0->1->2->3 puts 2 at depth 2
+    e = np.array([[0, 1], [1, 2], [2, 3], [0, 2]])
+    hg = fc.csr_from_edges(e, 4)
+    ok, _, d = hg.validate_bfs(0, np.array([0, 0, 0, 2], np.int32))
+    assert ok and d.tolist() == [0, 1, 1, 2]
+    ok, msg, _ = hg.validate_bfs(0, np.array([0, 0, 1, 2], np.int32))
+    assert not ok and "too far apart" in msg
