@@ -101,6 +101,9 @@ _sig("fc_gen_rmat", C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_do
      C.POINTER(_HostGraph))
 _sig("fc_gen_grid", C.c_int, C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.POINTER(_HostGraph))
 _sig("fc_gen_er", C.c_int, C.c_int64, C.c_double, C.c_uint64, C.POINTER(_HostGraph))
+_sig("fc_gen_rmat_gpu", C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
+     C.POINTER(_HostGraph))
+_sig("fc_csr_from_edges_gpu", C.c_int, _i64p, C.c_int64, C.c_int64, C.POINTER(_HostGraph))
 _sig("fc_sample_roots", C.c_int, C.POINTER(_HostGraph), C.c_int32, C.c_uint64, _i32p)
 _sig("fc_components", C.c_int, C.POINTER(_HostGraph), _i32p)
 _sig("fc_graph_create_csr32", C.c_int, _i32p, _i32p, C.c_int32, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p))
@@ -134,7 +137,8 @@ EXPORTED = [
     "fc_graph_create_csr32", "fc_graph_create_csr64", "fc_graph_destroy", "fc_bfs", "fc_graph_levels",
     "fc_graph_parents", "fc_stcon", "fc_stcon_many", "fc_stcon_sched_many", "fc_stcon_batch",
     "fc_sched_trace", "fc_sched_train", "fc_sched_set_params", "fc_sched_get_params", "fc_sched_forward",
-    "fc_sched_gradient", "fc_stcon_path", "fc_validate_stcon", "fc_validate_bfs",
+    "fc_sched_gradient", "fc_stcon_path", "fc_validate_stcon", "fc_validate_bfs", "fc_gen_rmat_gpu",
+    "fc_csr_from_edges_gpu",
 ]
 
 
@@ -241,6 +245,22 @@ def csr_from_edges(edges: np.ndarray, V: int = 0) -> HostGraph:
     e = np.ascontiguousarray(edges, dtype=np.int64).reshape(-1)
     hg = _HostGraph()
     _check(_lib.fc_csr_from_edges(_p64(e), len(e) // 2, V, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def gen_rmat_gpu(scale: int, edgefactor: float = 16.0, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                 seed: int = 1) -> HostGraph:
+    """gen_rmat with generation and CSR build on the GPU (same graph)."""
+    hg = _HostGraph()
+    _check(_lib.fc_gen_rmat_gpu(scale, edgefactor, a, b, c, seed, C.byref(hg)))
+    return HostGraph._take(hg)
+
+
+def csr_from_edges_gpu(edges: np.ndarray, V: int = 0) -> HostGraph:
+    """csr_from_edges with the CSR built on the GPU (same arrays)."""
+    e = np.ascontiguousarray(edges, dtype=np.int64).reshape(-1)
+    hg = _HostGraph()
+    _check(_lib.fc_csr_from_edges_gpu(_p64(e), len(e) // 2, V, C.byref(hg)))
     return HostGraph._take(hg)
 
 

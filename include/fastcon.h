@@ -128,6 +128,18 @@ fc_status fc_gen_er(int64_t n, double mean_degree, uint64_t seed, fc_host_graph*
  * engine's own stream). Returns FC_ERR_INVALID if fewer exist. */
 fc_status fc_sample_roots(const fc_host_graph* g, int32_t n, uint64_t seed, int32_t* out);
 
+/* The same two builders on the GPU, with the same results bit for bit: the
+ * edges are generated on the device (R-MAT) or uploaded, and the canonical
+ * CSR is built there by an in-house LSD radix sort on (u, v) keys, duplicate
+ * removal and scans, then copied back. Replaces make_struct's Thrust sorts
+ * and unique (st-conn/make_struct_gpufunc.cu:242-328,
+ * st-conn/cudakernel/gstruct_kernel.cu:114-145,187-235). Limits: 2*m below
+ * 2^32 - 2^13 (E fits 32-bit positions); needs about 20 bytes of device
+ * memory per directed edge. */
+fc_status fc_gen_rmat_gpu(int scale, double edgefactor, double a, double b, double c,
+                          uint64_t seed, fc_host_graph* out);
+fc_status fc_csr_from_edges_gpu(const int64_t* edges, int64_t m, int64_t V, fc_host_graph* out);
+
 /* Connected-component labels (label = smallest vertex id of the component). */
 fc_status fc_components(const fc_host_graph* g, int32_t* label_out);
 
