@@ -115,6 +115,7 @@ _sig("fc_graph_parents", C.c_int, C.c_void_p, _i32p)
 _sig("fc_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.POINTER(Stats))
 _sig("fc_stcon_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(Stats))
 _sig("fc_stcon_path", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int32, _i32p, C.POINTER(Stats))
+_sig("fc_stcon_seeds", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _i32p, C.POINTER(Stats))
 _sig("fc_validate_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32)
 _sig("fc_validate_bfs", C.c_int, C.POINTER(_HostGraph), C.c_int32, _i32p, C.c_int64, _i32p)
 _sig("fc_stcon_sched_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
@@ -138,7 +139,7 @@ EXPORTED = [
     "fc_graph_parents", "fc_stcon", "fc_stcon_many", "fc_stcon_sched_many", "fc_stcon_batch",
     "fc_sched_trace", "fc_sched_train", "fc_sched_set_params", "fc_sched_get_params", "fc_sched_forward",
     "fc_sched_gradient", "fc_stcon_path", "fc_validate_stcon", "fc_validate_bfs", "fc_gen_rmat_gpu",
-    "fc_csr_from_edges_gpu",
+    "fc_csr_from_edges_gpu", "fc_stcon_seeds",
 ]
 
 
@@ -362,6 +363,13 @@ class Graph:
             _check(code)
             return out[: n.value].copy(), st.as_dict()
         _check(code)
+
+    def stcon_seeds(self, s: int, t: int, helpers: int = 318, seed: int = 1):
+        """Helper-seed verdict (fc_stcon_seeds): (connected, stats)."""
+        conn = C.c_int32()
+        st = Stats()
+        _check(_lib.fc_stcon_seeds(self._h, int(s), int(t), int(helpers), int(seed), C.byref(conn), C.byref(st)))
+        return bool(conn.value), st.as_dict()
 
     def validate_stcon(self, s: int, t: int, mn: int, depth: int):
         """ST-CON's matching-node check (fc_validate_stcon): (ok, message)."""

@@ -231,6 +231,18 @@ fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t
 fc_status fc_stcon_path(fc_graph* g, int32_t s, int32_t t, int32_t* path_out, int32_t cap,
                         int32_t* len_out, fc_stats* stats);
 
+/* Helper-seed s–t connectivity (BFS_4K_MULTI,
+ * BFS_4K_MULTI/src/device/BFS_WorkEfficient.cu:9-168 with the contact
+ * closure of .../WorkEfficientKernel/BFS_WE_AdjMatrixAnalysis.cu:1-35):
+ * s, t and `helpers` random seeds (0..510, distinct, from `seed`) each grow
+ * a colour in one shared frontier; contacts between colours are closed
+ * from s's colour after every level. connected: 1 when the closure reaches
+ * t's colour, 0 when the frontier empties first. A verdict only: the
+ * chain of helper balls is no shortest path. stats: time_ms, levels,
+ * visited_vertices. */
+fc_status fc_stcon_seeds(fc_graph* g, int32_t s, int32_t t, int32_t helpers, uint64_t seed,
+                         int32_t* connected, fc_stats* stats);
+
 /* ST-CON's check of a matching node mn (validate_stcon_bfs,
  * st-conn/make_stcon.c:1578-1641): a BFS rooted at mn; following its tree
  * parents from t and from s must reach mn within depth steps each, and the
