@@ -333,6 +333,33 @@ constexpr int kBuLane = 128;
 // c's current frontier bitmap. In s–t modes the vertices of the other colour's
 // current frontier are scanned too, so that a frontier-to-frontier edge is
 // still seen as a meeting edge.
+// Per-chunk loads of a bottom-up step (64 consecutive vertices of one wave):
+// the candidate flags and the offset pair. They are issued for the next
+// chunk before the current chunk's columns are scanned, so a chunk costs
+// about two dependent memory round trips instead of three.
+template <typename OffT>
+struct BuChunk {
+  bool unvis, oth;
+  OffT b, e;
+};
+
+template <typename OffT>
+__device__ __forceinline__ BuChunk<OffT> bu_load(const VisitParams<OffT>& p, const uint32_t* ofbm, uint32_t w) {
+  const int lane = lane_id();
+  const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
+  const bool inr = v < p.V;
+  const uint32_t wi = 2u * w + ((uint32_t)lane >> 5);
+  BuChunk<OffT> c;
+  c.unvis = inr && !((p.visited[wi] >> (lane & 31)) & 1u);
+  c.oth = inr && ofbm && ((ofbm[wi] >> (lane & 31)) & 1u);
+  c.b = c.e = 0;
+  if (c.unvis || c.oth) {
+    c.b = p.off[v];
+    c.e = p.off[v + 1];
+  }
+  return c;
+}
+
 template <typename OffT>
 __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool st, int lvl, const uint32_t* cfbm,
                         const uint32_t* ofbm, int gwave, int nwaves) {
@@ -342,18 +369,18 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool
   const int lane = lane_id();
   const uint32_t nw64 = ((uint32_t)p.V + 63u) >> 6;
   const int32_t mine = st ? ((c ? kColorBit : 0) | (lvl + 1)) : lvl + 1;
-  for (uint32_t w = (uint32_t)gwave; w < nw64; w += (uint32_t)nwaves) {
+  uint32_t w = (uint32_t)gwave;
+  if (w >= nw64) return;
+  BuChunk<OffT> cur = bu_load(p, ofbm, w);
+  for (;;) {
+    const uint32_t wn = w + (uint32_t)nwaves;
+    const bool more = wn < nw64;
+    BuChunk<OffT> nx;
+    if (more) nx = bu_load(p, ofbm, wn);
     const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
-    const bool inr = v < p.V;
-    const uint32_t wi = 2u * w + ((uint32_t)lane >> 5);
-    const bool unvis = inr && !((p.visited[wi] >> (lane & 31)) & 1u);
-    const bool oth = inr && ofbm && ((ofbm[wi] >> (lane & 31)) & 1u);
+    const bool unvis = cur.unvis, oth = cur.oth;
     const bool cand = unvis || oth;
-    OffT b = 0, e = 0;
-    if (cand) {
-      b = p.off[v];
-      e = p.off[v + 1];
-    }
+    const OffT b = cur.b, e = cur.e;
     const bool small = cand && (e - b) <= (OffT)kBuLane;
     int32_t par = -1;
     if (small) {
@@ -412,7 +439,8 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool
       a.best = cnd < a.best ? cnd : a.best;
     }
     // Bitmap words of these 64 vertices belong to this wave during a
-    // bottom-up level, so they are updated without atomics.
+    // bottom-up level, so they are updated without atomics (and the
+    // prefetched words of the wave's next chunk are not changed by others).
     const uint64_t cm = ballot(claimed);
     if (lane == 0 && (uint32_t)cm) {
       p.visited[2u * w] |= (uint32_t)cm;
@@ -423,6 +451,9 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool
       nfbm[2u * w + 1u] = (uint32_t)(cm >> 32);
     }
     push_next(a, claimed, v, claimed ? (int64_t)(e - b) : 0, false);
+    if (!more) break;
+    cur = nx;
+    w = wn;
   }
 }
 
