@@ -1320,9 +1320,9 @@ static fc_status batch_alloc_lane(fc_graph* g, BatchWorkspace** out, bool own_st
     batch_free(b);
     return fail(FC_ERR_NOGPU, "batch kernel has no occupancy on this device");
   }
-  // With three concurrent lanes, 2-3 blocks per CU per lane measured best
-  // at s22 (1: 8.9 ms, 2: 7.8, 3: 7.8, 4: 8.0 for 1,024 queries).
-  int per_cu = std::min(occ, 3);
+  // With three concurrent lanes, 2 blocks per CU per lane measured best at
+  // s22 (1: 6.19 ms, 2: 5.58, 3: 5.79, 4: 6.26 for 1,024 queries).
+  int per_cu = std::min(occ, 2);
   const char* bpc = std::getenv("FASTCON_BATCH_BLOCKS_PER_CU");
   if (bpc && std::atoi(bpc) > 0) per_cu = std::min(occ, std::atoi(bpc));
   b->blocks = per_cu * g->cus;
