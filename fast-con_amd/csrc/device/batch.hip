@@ -44,7 +44,8 @@ constexpr float kDefaultLearnMargin = 0.1f;
 constexpr int kMaxLanes = 4;          // concurrent wave machines per batch
 constexpr int kDefaultLanes = 3;
 // A filter launch pays when the expected number of dropped list entries or
-// frontier edges reaches these.
+// frontier edges reaches these (defaults; FASTCON_FILTER_MIN_VERTICES and
+// FASTCON_FILTER_MIN_EDGES override them).
 constexpr uint64_t kFilterMinVertices = 1u << 15;
 constexpr uint64_t kFilterMinEdges = 1u << 20;
 
@@ -683,7 +684,8 @@ __device__ __forceinline__ void batch_apply_action(BatchDecision& d, int a) {
 }
 
 // Heuristic: expand both sides while the two frontiers together have at most
-// this many edges, about the work an extra step's fixed cost would buy.
+// this many edges, about the work an extra step's fixed cost would buy
+// (default; FASTCON_BOTH_MAX_EDGES overrides it).
 constexpr uint64_t kBothMaxEdges = 1u << 20;
 
 // Degree sum of side x's current frontier restricted to the live queries,
@@ -705,7 +707,7 @@ __device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t
   batch_features(ctl, mf, V, E);
   for (int k = 0; k < 12; ++k) f[k] = ctl->feat[k];
   if (ctl->policy == 0) return 2;  // plain: both sides top-down
-  if (mf[0] + mf[1] <= kBothMaxEdges) return 2;
+  if (mf[0] + mf[1] <= ctl->both_max_e) return 2;
   const int side = mf[0] <= mf[1] ? 0 : 1;
   // A bit-parallel bottom-up vertex stops only once it has found every live
   // query's bit: worth it only when the top-down frontier would touch a large
@@ -741,7 +743,7 @@ __device__ __forceinline__ void batch_apply(BatchCtl* ctl, int a, bool allow_fil
       if (!dead || !built) continue;
       const double frac = (double)dead / (double)built;
       const double size = (double)ctl->fl_n[x][c] + (double)(ctl->hub[x][c] >> kHubEdgeBits);
-      if (size * frac >= (double)kFilterMinVertices || (double)ctl->fl_mf[x][c] * frac >= (double)kFilterMinEdges)
+      if (size * frac >= (double)ctl->filt_min_v || (double)ctl->fl_mf[x][c] * frac >= (double)ctl->filt_min_e)
         filt |= 1 << x;
     }
   }
@@ -1459,6 +1461,13 @@ fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int
   init.trace_every = every;
   init.learn_margin_milli = (int32_t)std::lround((margin < 0.f ? kDefaultLearnMargin : margin) * 1000.f);
   init.f_branch = g->f_branch;
+  auto env_u64 = [](const char* name, uint64_t dflt) {
+    const char* v = std::getenv(name);
+    return v && std::strtoull(v, nullptr, 10) > 0 ? (uint64_t)std::strtoull(v, nullptr, 10) : dflt;
+  };
+  init.filt_min_v = env_u64("FASTCON_FILTER_MIN_VERTICES", kFilterMinVertices);
+  init.filt_min_e = env_u64("FASTCON_FILTER_MIN_EDGES", kFilterMinEdges);
+  init.both_max_e = env_u64("FASTCON_BOTH_MAX_EDGES", kBothMaxEdges);
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error

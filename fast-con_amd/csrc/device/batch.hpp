@@ -78,6 +78,9 @@ struct BatchCtl {
   uint32_t fl_n[2][2];     // normal frontier list sizes [side][parity]
   unsigned long long hub[2][2];   // hub lists: (entries << 40) | edges
   unsigned long long fl_mf[2][2]; // degree sums of the frontier (normal + hub)
+  unsigned long long filt_min_v;  // a filter runs when it drops at least this many list entries
+  unsigned long long filt_min_e;  //   or this many frontier edges
+  unsigned long long both_max_e;  // heuristic: both sides while their frontiers have at most this many edges
   uint32_t ticket;
   uint32_t err;
   // statistics (whole batch)
