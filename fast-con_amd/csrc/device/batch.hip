@@ -767,29 +767,42 @@ __device__ __forceinline__ int batch_next(BatchCtl* ctl, bool allow_filter) {
   return allow_filter ? kFinDecideFilter : kFinDecide;
 }
 
-template <typename OffT>
-__device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int phase, const BatchDecision& d,
-                                              const RedTot& r) {
-  BatchCtl* ctl = p.ctl;
-  const int wave0 = ctl->wave * 64;
-  // Counters the launch updated with atomics, all loads issued together.
-  uint32_t ln[2] = {0, 0};
-  unsigned long long lh[2] = {0, 0}, lm[2] = {0, 0}, act0 = 0;
+// Counters a launch updated with atomics, as the finalisation needs them:
+// INIT's lists at parity 0 (and the live mask), the other phases' next
+// lists. Loaded by the whole finalising wave (uniform addresses) before the
+// slot sums, so both sets of loads share one round trip.
+struct FinPre {
+  uint32_t ln[2];
+  unsigned long long lh[2], lm[2], act0;
+};
+
+__device__ __forceinline__ FinPre fin_prefetch(BatchCtl* ctl, int phase, const int* nxt) {
+  FinPre f{{0u, 0u}, {0ull, 0ull}, {0ull, 0ull}, 0ull};
   if (phase == kBInit) {
-    act0 = ld_agent(&ctl->act);
+    f.act0 = ld_agent(&ctl->act);
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      ln[x] = ld_agent(&ctl->fl_n[x][0]);
-      lh[x] = ld_agent(&ctl->hub[x][0]);
-      lm[x] = ld_agent(&ctl->fl_mf[x][0]);
+      f.ln[x] = ld_agent(&ctl->fl_n[x][0]);
+      f.lh[x] = ld_agent(&ctl->hub[x][0]);
+      f.lm[x] = ld_agent(&ctl->fl_mf[x][0]);
     }
   } else if (phase == kBCommit || phase == kBBottomUp || phase == kBFilter) {
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      ln[x] = ld_agent(&ctl->fl_n[x][ctl->cur[x] ^ 1]);
-      lh[x] = ld_agent(&ctl->hub[x][ctl->cur[x] ^ 1]);
+      f.ln[x] = ld_agent(&ctl->fl_n[x][nxt[x]]);
+      f.lh[x] = ld_agent(&ctl->hub[x][nxt[x]]);
     }
   }
+  return f;
+}
+
+template <typename OffT>
+__device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int phase, const BatchDecision& d,
+                                              const RedTot& r, const FinPre& f) {
+  BatchCtl* ctl = p.ctl;
+  const int wave0 = ctl->wave * 64;
+  const uint32_t ln[2] = {f.ln[0], f.ln[1]};
+  const unsigned long long lh[2] = {f.lh[0], f.lh[1]}, lm[2] = {f.lm[0], f.lm[1]}, act0 = f.act0;
   ctl->active_vertices += (int64_t)r.v[kRActV];
   ctl->active_edges += (int64_t)r.v[kRActE];
   ctl->touched_total += (int64_t)r.v[kRTouched];
@@ -1146,6 +1159,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   }
   int need = kFinDone;
   {
+    const FinPre pre = fin_prefetch(ctl, phase, nxt);
     const RedTot tot = collect_slots(p.red);
     if (threadIdx.x == 0) {
       if (ctl->phases < kPhaseLog) {
@@ -1157,7 +1171,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       if (phase == kBDecide)
         need = ctl->pause_filter ? kFinDecideFilter : kFinDecide;
       else
-        need = batch_finalize(p, phase, d, tot);
+        need = batch_finalize(p, phase, d, tot, pre);
     }
   }
   need = __shfl(need, 0);
