@@ -95,7 +95,6 @@ static VisitParams<OffT> make_params(fc_graph* g, int force_action) {
   p.qs = g->d_qs;
   p.qt = g->d_qt;
   p.out = g->d_out;
-  p.mlp = g->d_mlp;
   p.force_action = force_action;
   return p;
 }

@@ -498,7 +498,7 @@ __device__ __forceinline__ void apply_action(Decision& d, int a) {
 }
 
 // Decision for the next level launch (one thread of the finalising block).
-__device__ void decide(VisitCtl* ctl, const float* mlp, int64_t V, int64_t E) {
+__device__ void decide(VisitCtl* ctl, int64_t V, int64_t E) {
   Decision d;
   d.ex[0] = d.ex[1] = 0;
   d.strat[0] = d.strat[1] = kTopDown;
@@ -572,7 +572,7 @@ __device__ void finalize_reset(const VisitParams<OffT>& p) {
   ctl->visited_v = st ? 2 : 1;
   ctl->visited_e = (int64_t)seen;
   ctl->expv = ctl->expe = 0;
-  decide(ctl, p.mlp, p.V, p.E);
+  decide(ctl, p.V, p.E);
   ctl->phase = kPhaseLevel;
 }
 
@@ -636,7 +636,7 @@ __device__ void finalize_level(const VisitParams<OffT>& p, const Decision& d) {
     return;
   }
   for (int c = 0; c < 2; ++c) ctl->ring[c][(ctl->lvl[c] + 1) % 3] = Ring{0u, 0u, 0ull, 0ull};
-  decide(ctl, p.mlp, p.V, p.E);
+  decide(ctl, p.V, p.E);
 }
 
 template <typename OffT>

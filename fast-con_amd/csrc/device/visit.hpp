@@ -110,7 +110,6 @@ struct VisitParams {
   const int32_t* qs;        // query sources (device), nq entries
   const int32_t* qt;        // query targets
   VisitOut* out;            // results (device), nq entries
-  const float* mlp;         // kMlpParams scheduler weights (device)
   int32_t force_action;     // -1, or a scheduler action applied to this launch (tracing)
   int32_t pad;
 };
