@@ -4,9 +4,9 @@
 // once per wave instead of once per query.
 //
 // Per vertex u and side x (0 = s side, 1 = t side):
-//   rec[4u+2x]   queries whose side-x ball contains u
-//   rec[4u+2x+1] masks ORed in during a top-down expansion, consumed by commit
-//   F[x][p][u]   queries for which u is in side x's current frontier
+//   rec[2Vx+2u]   queries whose side-x ball contains u
+//   rec[2Vx+2u+1] masks ORed in during a top-down expansion, consumed by commit
+//   F[x][p][u]    queries for which u is in side x's current frontier
 // A step expands one side (heuristic / learned schedules) or both sides
 // (plain schedule). With i and j the levels both sides had reached before the
 // step, the balls of an unresolved query are disjoint, so its distance is
@@ -237,11 +237,19 @@ __device__ __forceinline__ void appends_block_flush(const BatchParams<OffT>& p, 
 // same value, so the race is benign). Every vertex whose N word becomes
 // non-zero is in a marked group: the first atomic on it is issued by a lane
 // whose earlier load saw zero.
+// Side x's {visited, N} pair of vertex u: the records are side-major, so a
+// one-side top-down probe reads 16 bytes of an array in which every record
+// belongs to that side.
+template <typename OffT>
+__device__ __forceinline__ unsigned long long* rec_of(const BatchParams<OffT>& p, int x, int64_t u) {
+  return p.rec + 2 * (int64_t)p.V * x + 2 * u;
+}
+
 template <typename OffT>
 __device__ __forceinline__ ulonglong2 probe_load(const BatchParams<OffT>& p, int x, unsigned long long fm, int32_t u,
                                                  bool act) {
   ulonglong2 vn = make_ulonglong2(~0ull, 0ull);
-  if (act && fm) vn = *reinterpret_cast<const ulonglong2*>(p.rec + 4 * (int64_t)u + 2 * x);  // {visited, next}
+  if (act && fm) vn = *reinterpret_cast<const ulonglong2*>(rec_of(p, x, u));  // {visited, next}
   return vn;
 }
 
@@ -250,7 +258,7 @@ __device__ __forceinline__ void probe_apply(const BatchParams<OffT>& p, int x, u
                                             ulonglong2 vn) {
   const unsigned long long m = fm & ~vn.x;
   if (m & ~vn.y) {
-    __hip_atomic_fetch_or(p.rec + 4 * (int64_t)u + 2 * x + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(rec_of(p, x, u) + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (vn.y == 0) p.tm[u >> 5] = 1;
   }
 }
@@ -450,9 +458,8 @@ __device__ __forceinline__ CommitLoad<OffT> commit_load(const BatchParams<OffT>&
   c.a = c.b = make_ulonglong2(0ull, 0ull);
   c.o0 = c.o1 = 0;
   if (c.valid) {
-    const ulonglong2* r = reinterpret_cast<const ulonglong2*>(p.rec + 4 * u);
-    c.a = r[0];
-    c.b = r[1];
+    c.a = *reinterpret_cast<const ulonglong2*>(rec_of(p, 0, u));
+    c.b = *reinterpret_cast<const ulonglong2*>(rec_of(p, 1, u));
     c.o0 = p.off[u];
     c.o1 = p.off[u + 1];
   }
@@ -503,9 +510,8 @@ __device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends&
         red[kRAnyS] |= nS;
         red[kRAnyT] |= nT;
         if (cur.a.y | cur.b.y) red[kRTouched] += 1;
-        ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
-        if (cur.a.y) r[0] = make_ulonglong2(vS | nS, 0ull);
-        if (cur.b.y) r[1] = make_ulonglong2(vT | nT, 0ull);
+        if (cur.a.y) *reinterpret_cast<ulonglong2*>(rec_of(p, 0, u)) = make_ulonglong2(vS | nS, 0ull);
+        if (cur.b.y) *reinterpret_cast<ulonglong2*>(rec_of(p, 1, u)) = make_ulonglong2(vT | nT, 0ull);
         if (nS) p.F[0][nxt[0]][u] = nS;
         if (nT) p.F[1][nxt[1]][u] = nT;
         deg = (int64_t)(cur.o1 - cur.o0);
@@ -544,9 +550,8 @@ __device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& a
     unsigned long long accS = 0, accT = 0, needS = 0, needT = 0, vS = 0, vT = 0;
     OffT b = 0, e = 0;
     if (u < p.V) {
-      const ulonglong2* r = reinterpret_cast<const ulonglong2*>(p.rec + 4 * (int64_t)u);
-      vS = r[0].x;
-      vT = r[1].x;
+      vS = rec_of(p, 0, u)[0];
+      vT = rec_of(p, 1, u)[0];
       if (DS) needS = act & ~vS;
       if (DT) needT = act & ~vT;
       if (needS | needT) {
@@ -575,9 +580,8 @@ __device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& a
     const unsigned long long nS = accS & needS, nT = accT & needT;
     int64_t deg = 0;
     if (nS | nT) {
-      ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * (int64_t)u);
-      if (nS) r[0].x = vS | nS;
-      if (nT) r[1].x = vT | nT;
+      if (nS) rec_of(p, 0, u)[0] = vS | nS;
+      if (nT) rec_of(p, 1, u)[0] = vT | nT;
       if (nS) p.F[0][nxt[0]][u] = nS;
       if (nT) p.F[1][nxt[1]][u] = nT;
       red[kROdd] |= (nS & vT) | (nT & vS);
@@ -956,7 +960,7 @@ __device__ __forceinline__ void init_wave(const BatchParams<OffT>& p, int32_t* s
   }
   int64_t deg = 0;
   if (owner) {
-    p.rec[4 * (int64_t)u + 2 * x] = mask;
+    rec_of(p, x, u)[0] = mask;
     p.F[x][0][u] = mask;
     atomicOr(&p.db[u >> 5], 1u << (u & 31));
     deg = bdeg(p, u);
@@ -1099,9 +1103,8 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
           const uint32_t wv = (uint32_t)__shfl((int)word, mine < 0 ? 0 : mine);
           const int64_t u = gd(mine < 0 ? 0 : mine) * 32 + (lane & 31);
           if (mine >= 0 && ((wv >> (lane & 31)) & 1u) && u < p.V) {
-            ulonglong2* r = reinterpret_cast<ulonglong2*>(p.rec + 4 * u);
-            r[0] = make_ulonglong2(0ull, 0ull);
-            r[1] = make_ulonglong2(0ull, 0ull);
+            *reinterpret_cast<ulonglong2*>(rec_of(p, 0, u)) = make_ulonglong2(0ull, 0ull);
+            *reinterpret_cast<ulonglong2*>(rec_of(p, 1, u)) = make_ulonglong2(0ull, 0ull);
             p.F[0][0][u] = 0ull;
             p.F[0][1][u] = 0ull;
             p.F[1][0][u] = 0ull;

@@ -103,9 +103,10 @@ struct BatchParams {
   int32_t V;
   int32_t hub_threshold;
   int64_t E;
-  // Per-vertex record of four 64-bit words, rec[4u + 2x] = side-x visited
-  // mask and rec[4u + 2x + 1] = side-x masks ORed in during the current
-  // top-down step: one 16-byte load gives a probe both words it needs.
+  // Per-side records of two 64-bit words, side-major: rec[2V*x + 2u] =
+  // side-x visited mask of u and rec[2V*x + 2u + 1] = side-x masks ORed in
+  // during the current top-down step. One 16-byte load gives a probe both
+  // words it needs, and a line holds eight vertices' records of one side.
   unsigned long long* rec;     // [4*V]
   unsigned long long* F[2][2]; // frontier masks [side][parity], V entries each
   uint8_t* tm;                 // touched marks: one byte per 32 vertices whose N words may be non-zero
