@@ -16,8 +16,9 @@
    learned schedule itself, on new queries, and the network is refitted on
    both sets of rows (one round of dataset aggregation, so that it also sees
    the states its own choices lead to).
-   The margin by which the network must beat the heuristic's action is
-   chosen on validation graphs (total regret over a grid of margins).
+   The final fit's epoch count and the margin by which the network must
+   beat the heuristic's action are chosen on validation graphs (lowest
+   total regret over a grid of both).
 3. Report, on the training rows and on held-out graphs and queries:
    regret (cost of the chosen action over the best action, per traced step)
    of the network, the heuristic and the plain both-sides schedule; and the
@@ -150,6 +151,8 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--repeats", type=int, default=3, help="timed rollouts per traced state (median)")
+    ap.add_argument("--epochs-grid", default="200,600,2000",
+                    help="final-fit epoch counts tried; the one with the lowest validation regret is kept")
     args = ap.parse_args()
     train, valid, held = family(args.quick)
 
@@ -174,14 +177,8 @@ def main():
         F.append(f), Cst.append(c), Hh.append(h)
     F, Cst, Hh = np.concatenate(F), np.concatenate(Cst), np.concatenate(Hh)
 
-    log(f"train on {len(F)} rows ({args.epochs} epochs, lr {args.lr})")
-    t0 = time.time()
-    loss = g0.sched_train(F, Cst, epochs=args.epochs, lr=args.lr, seed=args.seed)
-    w = g0.sched_get_params()
-    log(f"  final loss {loss:.4f} ({time.time() - t0:.1f}s)")
-
-    # margin: the one with the lowest total regret on the validation graphs
-    log("validation graphs (margin choice)")
+    # validation rows (heuristic schedule) for the choice of epochs and margin
+    log("validation graphs")
     VF, VC, VH = [], [], []
     for i, (name, build, pairs, every, cap) in enumerate(valid):
         g, hg = open_graph(build, 350 + i)
@@ -189,12 +186,26 @@ def main():
         VF.append(f), VC.append(c), VH.append(h)
         g.close()
     VF, VC, VH = np.concatenate(VF), np.concatenate(VC), np.concatenate(VH)
-    vpred = g0.sched_forward(VF)
     margins = [0.0, 0.02, 0.05, 0.1, 0.15, 0.2, 0.3, 0.5, 1.0]
-    vreg = {m: regret(VC, pick_actions(vpred, VH, m))[1] for m in margins}
-    margin = min(margins, key=lambda m: (round(vreg[m], 4), -m))
-    log("  total regret by margin " + json.dumps({str(m): round(v, 4) for m, v in vreg.items()}) +
-        f"; heuristic {regret(VC, VH)[1]:.4f}; chosen {margin}")
+    best = None
+    grid = {}
+    for ep in [int(x) for x in args.epochs_grid.split(",") if x.strip()]:
+        t0 = time.time()
+        loss_ep = g0.sched_train(F, Cst, epochs=ep, lr=args.lr, seed=args.seed)
+        w_ep = g0.sched_get_params()
+        vpred = g0.sched_forward(VF)
+        vreg = {m: regret(VC, pick_actions(vpred, VH, m))[1] for m in margins}
+        m_ep = min(margins, key=lambda m: (round(vreg[m], 4), -m))
+        grid[str(ep)] = {"loss": round(float(loss_ep), 4), "margin": m_ep,
+                         "validation_total_regret_by_margin": {str(m): round(v, 4) for m, v in vreg.items()}}
+        log(f"  fit {ep} epochs on {len(F)} rows: loss {loss_ep:.4f} ({time.time() - t0:.1f}s); validation total "
+            f"regret by margin {json.dumps(grid[str(ep)]['validation_total_regret_by_margin'])}; best margin {m_ep}")
+        key = (round(vreg[m_ep], 4), -m_ep)
+        if best is None or key < best[0]:
+            best = (key, ep, m_ep, w_ep, loss_ep, vreg)
+    _, epochs, margin, w, loss, vreg = best
+    g0.sched_set_params(w)
+    log(f"chosen: {epochs} epochs, margin {margin}; heuristic validation regret {regret(VC, VH)[1]:.4f}")
 
     def report_rows(feats, cost, heur):
         pred = g0.sched_forward(feats)
@@ -207,9 +218,9 @@ def main():
         return rows
 
     report = {"train": report_rows(F, Cst, Hh), "validation": report_rows(VF, VC, VH), "held_out": {},
-              "end_to_end": {}, "loss": loss, "epochs": args.epochs, "lr": args.lr, "margin": margin,
+              "end_to_end": {}, "loss": float(loss), "epochs": epochs, "lr": args.lr, "margin": margin,
               "validation_total_regret_by_margin": {str(m): round(v, 4) for m, v in vreg.items()},
-              "repeats": args.repeats}
+              "epochs_grid": grid, "first_fit_epochs": args.epochs, "repeats": args.repeats}
     log("train-set regret " + json.dumps(report["train"]))
     for name, g, hg, pairs, every, cap in graphs:
         if g is not g0:
