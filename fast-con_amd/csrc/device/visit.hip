@@ -74,6 +74,7 @@ struct Appender {
   int nxt_par;
   Ring* nxt;
   uint32_t* nfbm;
+  bool bitmaps;                 // keep the visited and frontier bitmaps (bottom-up levels may follow)
   int32_t* buf;                 // this wave's LDS stage, kStageCap entries
   uint32_t cnt;                 // staged entries (wave-uniform)
   uint64_t mf;                  // per-lane degree sum of claimed vertices
@@ -212,7 +213,7 @@ __device__ __forceinline__ bool probe_td(Appender<OffT>& a, int32_t* parent, boo
   if (cur == -1) {
     cur = atomicCAS(&p.state[u], -1, mine);
     if (cur == -1) {
-      atomicOr(p.visited + (u >> 5), 1u << (u & 31));
+      if (a.bitmaps) atomicOr(p.visited + (u >> 5), 1u << (u & 31));
       if (parent) parent[u] = v;
       return true;
     }
@@ -260,8 +261,8 @@ __device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bo
       const int32_t u1 = a1 ? p.col[b + VW] : 0;
       const bool c0 = probe_td(a, parent, st, lvl, v, u0, a0);
       const bool c1 = probe_td(a, parent, st, lvl, v, u1, a1);
-      push_next(a, c0, u0, c0 ? degree_of(p, u0) : 0, true);
-      push_next(a, c1, u1, c1 ? degree_of(p, u1) : 0, true);
+      push_next(a, c0, u0, c0 ? degree_of(p, u0) : 0, a.bitmaps);
+      push_next(a, c1, u1, c1 ? degree_of(p, u1) : 0, a.bitmaps);
       b += 2 * VW;
     }
   }
@@ -320,7 +321,7 @@ __device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool
       }
       const bool ok = act && v >= 0;
       const bool cl = probe_td(a, parent, st, lvl, v, u, ok);
-      push_next(a, cl, u, cl ? degree_of(p, u) : 0, true);
+      push_next(a, cl, u, cl ? degree_of(p, u) : 0, a.bitmaps);
     }
   }
 }
@@ -651,6 +652,9 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
   const int64_t nthr = (int64_t)nb * kBlock;
   const uint32_t nwords = (((uint32_t)p.V + 63u) >> 6) * 2u;
   const bool st = ctl->mode != kModeBfs;
+  // The plain s–t visit runs top-down only: it claims through state alone
+  // and keeps neither the visited nor the frontier bitmaps.
+  const bool bitmaps = ctl->mode != kModePlainSt;
   int32_t* const parent = ctl->record_parents ? p.parent : nullptr;
   __shared__ int s_last;
   __shared__ int32_t s_stage[kWavesPerBlock * kStageCap];
@@ -669,7 +673,7 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
       p.state[v] = s;
       if (parent) parent[v] = s != -1 ? (int32_t)v : -1;
     }
-    for (int64_t w = gtid; w < nwords; w += nthr) {
+    for (int64_t w = bitmaps ? gtid : (int64_t)nwords; w < nwords; w += nthr) {
       const uint32_t bs = ((uint32_t)src >> 5) == (uint32_t)w ? 1u << (src & 31) : 0u;
       const uint32_t bt = (dst >= 0 && ((uint32_t)dst >> 5) == (uint32_t)w) ? 1u << (dst & 31) : 0u;
       p.visited[w] = bs | bt;
@@ -685,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
     const int lvl[2] = {ctl->lvl[0], ctl->lvl[1]};
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      if (!d.ex[c]) continue;
+      if (!d.ex[c] || !bitmaps) continue;
       uint32_t* far = p.fbm[c][(lvl[c] + 2) % 3];
       for (int64_t w = gtid; w < nwords; w += nthr) far[w] = 0u;
     }
@@ -700,6 +704,7 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
       a.nxt_par = cur_par ^ 1;
       a.nxt = &ctl->ring[c][(lvl[c] + 1) % 3];
       a.nfbm = p.fbm[c][(lvl[c] + 1) % 3];
+      a.bitmaps = bitmaps;
       a.buf = s_stage + (threadIdx.x >> 6) * kStageCap;
       a.cnt = 0;
       a.mf = 0;
