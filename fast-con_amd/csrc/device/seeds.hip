@@ -43,6 +43,7 @@ struct SeedsCtl {
   uint32_t err;
   int32_t pad;
   int64_t claimed;   // vertices claimed after the seeds
+  uint32_t reach[kSeedWords];  // colours joined to colour 0 so far (only grows)
 };
 
 template <typename OffT>
@@ -80,12 +81,39 @@ __global__ void seeds_start_kernel(SeedsParams<OffT> p, const int32_t* __restric
     c->qn[1] = 0u;
     c->err = 0u;
     c->claimed = 0;
+    for (int w = 0; w < kSeedWords; ++w) c->reach[w] = w == 0 ? 1u : 0u;
   }
+}
+
+// Claimed vertices are staged per wave in LDS and published with one atomic
+// on the frontier counter per full stage, instead of one per wave and edge.
+constexpr uint32_t kSeedStage = 512;
+
+struct SeedStage {
+  int32_t* buf;
+  uint32_t cnt;  // wave-uniform
+};
+
+__device__ __forceinline__ void seed_flush(SeedStage& st, int32_t* nq, uint32_t* nqn, uint32_t cap, uint32_t* err) {
+  if (!st.cnt) return;
+  __builtin_amdgcn_wave_barrier();
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(nqn, st.cnt);
+  base = (uint32_t)__shfl((int)base, 0);
+  for (uint32_t i = (uint32_t)lane_id(); i < st.cnt; i += kWave) {
+    if (base + i < cap)
+      nq[base + i] = st.buf[i];
+    else
+      atomicOr(err, 1u);
+  }
+  __builtin_amdgcn_wave_barrier();
+  st.cnt = 0;
 }
 
 template <typename OffT>
 __device__ __forceinline__ void seeds_probe(const SeedsParams<OffT>& p, int32_t u, int32_t mine, int32_t cv,
-                                            bool act, int32_t* nq, uint32_t* nqn, uint32_t& claimed) {
+                                            bool act, int32_t* nq, uint32_t* nqn, uint32_t& claimed,
+                                            SeedStage& st) {
   bool won = false;
   if (act) {
     int32_t cur = p.state[u];
@@ -105,15 +133,15 @@ __device__ __forceinline__ void seeds_probe(const SeedsParams<OffT>& p, int32_t 
       }
     }
   }
-  uint64_t m;
-  const uint32_t slot = wave_reserve(nqn, won, &m);
+  const uint64_t m = ballot(won);
+  if (!m) return;
+  const uint32_t k = (uint32_t)__popcll(m);
+  if (st.cnt + k > kSeedStage) seed_flush(st, nq, nqn, (uint32_t)p.V, &p.ctl->err);
   if (won) {
     claimed += 1;
-    if (slot < (uint32_t)p.V)
-      nq[slot] = u;
-    else
-      atomicOr(&p.ctl->err, 1u);
+    st.buf[st.cnt + lane_rank(m)] = u;
   }
+  st.cnt += k;
 }
 
 // Top-down level: a lane per frontier vertex for its own list when it has
@@ -132,6 +160,8 @@ __global__ __launch_bounds__(256) void seeds_level_kernel(SeedsParams<OffT> p) {
   uint32_t* nqn = &ctl->qn[par ^ 1];
   const int lane = lane_id();
   uint32_t claimed = 0;
+  __shared__ int32_t s_stage[256 / kWave][kSeedStage];
+  SeedStage st{s_stage[threadIdx.x >> 6], 0u};
   const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
   const int64_t nr = ((int64_t)n + kWave - 1) / kWave * kWave;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += nthr) {
@@ -155,7 +185,7 @@ __global__ __launch_bounds__(256) void seeds_level_kernel(SeedsParams<OffT> p) {
     while (ballot(j < e)) {
       const bool act = j < e;
       const int32_t u = act ? p.col[j] : 0;
-      seeds_probe(p, u, mine, cv, act, nq, nqn, claimed);
+      seeds_probe(p, u, mine, cv, act, nq, nqn, claimed, st);
       ++j;
     }
     // heavy vertices: the whole wave per list
@@ -169,23 +199,26 @@ __global__ __launch_bounds__(256) void seeds_level_kernel(SeedsParams<OffT> p) {
         const OffT x = k + (OffT)lane;
         const bool act = x < he;
         const int32_t u = act ? p.col[x] : 0;
-        seeds_probe(p, u, hmine, hcv, act, nq, nqn, claimed);
+        seeds_probe(p, u, hmine, hcv, act, nq, nqn, claimed, st);
       }
     }
   }
+  seed_flush(st, nq, nqn, (uint32_t)p.V, &ctl->err);
   const uint64_t cl = wave_sum_u64(claimed);
   if (lane == 0 && cl) atomicAdd((unsigned long long*)&ctl->claimed, cl);
 }
 
 // One block after each level: close the contact relation from colour 0 and
 // decide; otherwise advance the level and restart the consumed queue's
-// counter, which the next level fills.
+// counter, which the next level fills. Contacts only accumulate, so the
+// closure resumes from the set the previous level reached and usually
+// settles in a round or two.
 __global__ __launch_bounds__(256) void seeds_check_kernel(uint32_t* __restrict__ contact, int K, SeedsCtl* ctl) {
   __shared__ uint32_t reach[kSeedWords];
   __shared__ int changed;
   if (ctl->done) return;
   const int tid = (int)threadIdx.x;
-  if (tid < kSeedWords) reach[tid] = tid == 0 ? 1u : 0u;
+  if (tid < kSeedWords) reach[tid] = ctl->reach[tid];
   __syncthreads();
   for (int round = 0; round <= K; ++round) {
     if (tid == 0) changed = 0;
@@ -205,6 +238,7 @@ __global__ __launch_bounds__(256) void seeds_check_kernel(uint32_t* __restrict__
     if (!changed) break;
     __syncthreads();
   }
+  if (tid < kSeedWords) ctl->reach[tid] = reach[tid];
   if (tid == 0) {
     const int32_t lvl = ctl->level;
     const uint32_t next = ctl->qn[(lvl & 1) ^ 1];
