@@ -53,3 +53,18 @@ def test_seed_levels_shrink_on_a_grid(fc, orc):
     say(f"grid256 corners: plain {st0['levels']} levels {st0['time_ms']:.2f} ms; "
         f"seeds {st1['levels']} levels {st1['time_ms']:.2f} ms")
     assert st1["levels"] * 4 < st0["levels"]
+
+
+def test_seed_tiny_graphs(fc, orc):
+    """More helpers than vertices, isolated vertices and an edgeless graph."""
+    cases = [
+        (fc.csr_from_edges(np.array([[0, 1]]), 2), [(0, 1, True)]),
+        (fc.csr_from_edges(np.array([[0, 1], [2, 3]]), 5), [(0, 1, True), (1, 2, False), (4, 0, False), (3, 2, True)]),
+        (fc.csr_from_edges(np.zeros((0, 2), np.int64), 4), [(0, 3, False), (2, 2, True)]),
+    ]
+    for hg, qs in cases:
+        with fc.Graph(hg) as g:
+            for s, t, exp in qs:
+                for helpers in (0, 5, 510):
+                    c, _ = g.stcon_seeds(s, t, helpers=helpers, seed=1)
+                    assert c == exp, (hg.V, s, t, helpers)
