@@ -681,9 +681,9 @@ __device__ __forceinline__ int bdegree_width(uint64_t n, uint64_t mf) {
 }
 
 __device__ __forceinline__ void batch_apply_action(BatchDecision& d, int a) {
-  const int k = a % 3;
+  const int k = a % 3, kind = a / 3;
   d.sides = k == 0 ? kSideS : k == 1 ? kSideT : kSideBoth;
-  d.strat = a < 3 ? kTopDown : kBottomUp;
+  d.strat = kind == 2 ? kBottomUp : kTopDown;
   d.action = a;
 }
 
@@ -716,7 +716,7 @@ __device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t
   // A bit-parallel bottom-up vertex stops only once it has found every live
   // query's bit: worth it only when the top-down frontier would touch a large
   // part of the graph anyway.
-  return side + ((double)mf[side] * 4.0 > (double)E ? 3 : 0);
+  return side + ((double)mf[side] * 4.0 > (double)E ? 6 : 0);
 }
 
 // Turn action a into the step decision and the next phase, filtering first
@@ -734,6 +734,7 @@ __device__ __forceinline__ void batch_apply(BatchCtl* ctl, int a, bool allow_fil
   }
   d.vw = bdegree_width(n, m);
   if (ctl->vw_fixed > 0) d.vw = ctl->vw_fixed;
+  if (a / 3 == 1) d.vw = 64;  // the wide top-down actions
   ctl->dec = d;
   int filt = 0;
   if (allow_filter) {

@@ -5,9 +5,9 @@
 
 namespace fcd {
 
-// Scheduler MLP (12 -> 32 -> 32 -> 6, ReLU); the output is a predicted
+// Scheduler MLP (12 -> 32 -> 32 -> 9, ReLU); the output is a predicted
 // log-cost per action. Canonical parameter layout (fc_sched_set_params):
-// W1[32][12] b1[32] W2[32][32] b2[32] W3[6][32] b3[6].
+// W1[32][12] b1[32] W2[32][32] b2[32] W3[9][32] b3[9].
 
 // Position of canonical parameter i in the wave-friendly layout used by
 // mlp_pick_wave_t: W1, W2 and W3 transposed (input-major), so that the lanes

@@ -1,4 +1,4 @@
-// Learned scheduler: the 12 -> 32 -> 32 -> 6 MLP that predicts, for each
+// Learned scheduler: the 12 -> 32 -> 32 -> 9 MLP that predicts, for each
 // batched-step action, the log of the time to finish the wave. Forward,
 // backward, gradient reduction and Adam are hand-written kernels; the batch
 // kernel evaluates the trained network (mlp.hpp) at every step decision.
@@ -8,7 +8,7 @@
 // reorders that work, so verdicts and distances are unchanged.
 //
 // Parameter layout (FC_SCHED_PARAMS floats):
-//   W1[32][12] b1[32] W2[32][32] b2[32] W3[6][32] b3[6].
+//   W1[32][12] b1[32] W2[32][32] b2[32] W3[9][32] b3[9].
 // Loss over n rows: L = 0.5 * sum_r sum_a (y[r][a] - target[r][a])^2 / n.
 #include "engine.hpp"
 

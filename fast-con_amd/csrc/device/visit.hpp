@@ -87,7 +87,7 @@ struct VisitCtl {
 
 constexpr int kFeatures = 12;
 constexpr int kHidden = 32;
-constexpr int kActions = 6;
+constexpr int kActions = 9;  // sides {s, t, both} x {top-down, top-down width 64, bottom-up}
 constexpr int kMlpParams = kFeatures * kHidden + kHidden + kHidden * kHidden + kHidden + kHidden * kActions + kActions;
 
 template <typename OffT>

@@ -33,8 +33,8 @@ POLICY_PLAIN, POLICY_HEURISTIC, POLICY_LEARNED = 0, 1, 2
 GRAPH_KEEP_ORDER = 1
 GRAPH_KEEP_IDS = 2
 DIR_FILE, DIR_DIRECTED, DIR_UNDIRECTED = 0, 1, 2
-SCHED_FEATURES, SCHED_HIDDEN, SCHED_ACTIONS = 12, 32, 6
-SCHED_PARAMS = 12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6
+SCHED_FEATURES, SCHED_HIDDEN, SCHED_ACTIONS = 12, 32, 9
+SCHED_PARAMS = 12 * 32 + 32 + 32 * 32 + 32 + 32 * SCHED_ACTIONS + SCHED_ACTIONS
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -414,7 +414,7 @@ class Graph:
     # ---- learned scheduler ----
     def sched_trace(self, s, t, cap: int = 4096, every: int = 1, policy: int = POLICY_HEURISTIC):
         """Trace the heuristic schedule over the queries. Returns (features
-        [rows, 12], cost_us [rows, 6], heuristic action [rows], wave [rows],
+        [rows, 12], cost_us [rows, 9], heuristic action [rows], wave [rows],
         distances [n])."""
         s = np.ascontiguousarray(s, np.int32)
         t = np.ascontiguousarray(t, np.int32)

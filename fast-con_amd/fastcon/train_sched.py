@@ -5,12 +5,13 @@
 1. Trace: on a seeded family of graphs (undirected R-MAT with the initiator
    a varied around 0.57 and edge factors 8 and 16, Erdős–Rényi with mean
    degrees 6 to 16, 2-D grids with 2% of the edges deleted) the engine runs seeded s–t batches along the heuristic
-   schedule and, at every traced step decision, times each of the six
-   actions (side s / t / both, top-down / bottom-up) from that state to the
+   schedule and, at every traced step decision, times each of the nine
+   actions (side s / t / both; top-down at the engine's width, top-down at
+   width 64, bottom-up) from that state to the
    end of the 64-query wave (fc_sched_trace).
    Each state is timed --repeats times (the same decisions each time) and
    the median cost is the label.
-2. Train: the 12 -> 32 -> 32 -> 6 network is fitted on the GPU to the
+2. Train: the 12 -> 32 -> 32 -> 9 network is fitted on the GPU to the
    centred log-costs by the engine's own forward/backward/Adam kernels
    (fc_sched_train). Then the training graphs are traced again along the
    learned schedule itself, on new queries, and the network is refitted on

@@ -299,22 +299,28 @@ fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t* t, int32_
                          fc_batch_stats* stats);
 
 /* ------------------------------------------------------------------------ */
-/* Learned scheduler: 12 frontier features -> 32 -> 32 -> 6 choices, FP32.   */
+/* Learned scheduler: 12 frontier features -> 32 -> 32 -> 9 choices, FP32.   */
 /* ------------------------------------------------------------------------ */
 
 #define FC_SCHED_FEATURES 12
 #define FC_SCHED_HIDDEN 32
-#define FC_SCHED_ACTIONS 6
-#define FC_SCHED_PARAMS (12 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6)
+#define FC_SCHED_ACTIONS 9
+#define FC_SCHED_PARAMS (12 * 32 + 32 + 32 * 32 + 32 + 32 * FC_SCHED_ACTIONS + FC_SCHED_ACTIONS)
 
-/* Scheduler actions of a batched step: sides {s, t, both}[a % 3], top-down
- * for a < 3, bottom-up for a >= 3. */
+/* Scheduler actions of a batched step: sides {s, t, both}[a % 3], and by
+ * a / 3 the direction and the virtual-warp width of top-down steps:
+ * 0 top-down with the engine's width (the autotuned one, else chosen from
+ * the frontier's mean degree), 1 top-down with a whole wave per frontier
+ * vertex (width 64), 2 bottom-up. */
 #define FC_ACTION_S_TOPDOWN 0
 #define FC_ACTION_T_TOPDOWN 1
 #define FC_ACTION_BOTH_TOPDOWN 2
-#define FC_ACTION_S_BOTTOMUP 3
-#define FC_ACTION_T_BOTTOMUP 4
-#define FC_ACTION_BOTH_BOTTOMUP 5
+#define FC_ACTION_S_TOPDOWN_WIDE 3
+#define FC_ACTION_T_TOPDOWN_WIDE 4
+#define FC_ACTION_BOTH_TOPDOWN_WIDE 5
+#define FC_ACTION_S_BOTTOMUP 6
+#define FC_ACTION_T_BOTTOMUP 7
+#define FC_ACTION_BOTH_BOTTOMUP 8
 
 /* Per-step trace record logged by timing every alternative from one state:
  * cost_us[a] is the device time from taking action a at this step to the

@@ -10,8 +10,8 @@ make_stcon.c:960-1200); it is restated here in float64 numpy:
   h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), y = W3 h2 + b3
   L  = 0.5 * sum_r sum_a (y[r,a] - target[r,a])^2 / n
 
-with the flat parameter layout W1[32][12] b1[32] W2[32][32] b2[32] W3[6][32]
-b3[6]. Pinning: the analytic gradient is checked against central finite
+with the flat parameter layout W1[32][12] b1[32] W2[32][32] b2[32] W3[9][32]
+b3[9]. Pinning: the analytic gradient is checked against central finite
 differences in float64 (tests/test_sched_oracle.py), the forward pass
 against hand-built networks with known outputs, and the initialisation and
 Adam step against their published definitions (Glorot & Bengio 2010 uniform
@@ -19,7 +19,7 @@ bound sqrt(6/(fan_in+fan_out)); Kingma & Ba 2015 with bias correction).
 """
 import numpy as np
 
-F, H, A = 12, 32, 6
+F, H, A = 12, 32, 9
 P = F * H + H + H * H + H + A * H + A
 _M64 = (1 << 64) - 1
 

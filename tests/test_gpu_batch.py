@@ -114,9 +114,10 @@ def _forced_params(fc, action):
     return w
 
 
-@pytest.mark.parametrize("action", range(6))
+@pytest.mark.parametrize("action", range(9))
 def test_batch_every_action_is_exact(fc, orc, action):
-    """Each scheduler action (side s / t / both, top-down / bottom-up) forced
+    """Each scheduler action (side s / t / both; top-down at the engine's
+    width, top-down at width 64, bottom-up) forced
     at every step still gives the oracle's verdicts and distances."""
     cases = [("rmat13", fc.gen_rmat(13, 8, seed=31)), ("er-sparse", fc.gen_er(3000, 1.3, 8)),
              ("grid", fc.gen_grid(60, 50, 0.04, 2))]

@@ -74,7 +74,8 @@ def test_training_reduces_loss(fc):
     rng = np.random.default_rng(8)
     X = rng.uniform(0, 1, (400, mlp.F)).astype(np.float32)
     # costs that depend on the features: action a is cheap when x[a] is small
-    cost = np.exp(2.0 * X[:, :mlp.A] + 0.1 * rng.normal(size=(400, mlp.A))).astype(np.float32)
+    clean = np.exp(2.0 * X[:, :mlp.A])
+    cost = (clean * np.exp(0.1 * rng.normal(size=(400, mlp.A)))).astype(np.float32)
     T = mlp.row_targets(cost)
     with fc.Graph(hg) as g:
         l1 = g.sched_train(X, cost, epochs=1, lr=3e-3, seed=2)
@@ -83,8 +84,12 @@ def test_training_reduces_loss(fc):
     assert l2 < 0.25 * l1
     lo, _ = mlp.loss_grad(w.astype(np.float64), X, T)
     assert abs(lo - l2) <= 1e-3 * max(1.0, lo)
+    # judged on the noise-free costs: with nine actions the label noise alone
+    # moves the noisy argmin off the true one on about a fifth of the rows
     pick = mlp.argmin_actions(w, X)
-    assert (pick == np.argmin(cost, 1)).mean() > 0.8
+    assert (pick == np.argmin(clean, 1)).mean() > 0.9
+    rows = np.arange(len(X))
+    assert np.mean(clean[rows, pick] / clean.min(axis=1) - 1.0) < 0.01
 
 
 def test_trace_keeps_answers_and_times_every_action(fc, orc):

@@ -54,7 +54,7 @@ def test_planted_mistakes_are_caught():
 
 
 def test_forward_hand_built_network():
-    # W1 = [I; 0], W2 = I, W3 = 2*[I 0], b3 = 0..5: y_a = 2*relu(x_a) + a
+    # W1 = [I; 0], W2 = I, W3 = 2*[I 0], b3 = 0..A-1: y_a = 2*relu(x_a) + a
     w = np.zeros(mlp.P)
     for j in range(mlp.F):
         w[j * mlp.F + j] = 1.0
@@ -67,7 +67,7 @@ def test_forward_hand_built_network():
     w[-mlp.A:] = np.arange(mlp.A)
     x = np.array([[0.5, -1.0, 0.25, 3.0, -2.0, 1.0, 0, 0, 0, 0, 0, 0]])
     y = mlp.forward(w, x)
-    assert np.allclose(y[0], 2 * np.maximum(x[0, :6], 0) + np.arange(6))
+    assert np.allclose(y[0], 2 * np.maximum(x[0, :mlp.A], 0) + np.arange(mlp.A))
     assert mlp.argmin_actions(w, x)[0] == int(np.argmin(y[0]))
 
 
