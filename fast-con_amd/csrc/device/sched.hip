@@ -9,7 +9,8 @@
 //
 // Parameter layout (FC_SCHED_PARAMS floats):
 //   W1[32][12] b1[32] W2[32][32] b2[32] W3[9][32] b3[9].
-// Loss over n rows: L = 0.5 * sum_r sum_a (y[r][a] - target[r][a])^2 / n.
+// Loss over n rows with row weights w_r (all 1 unless given):
+//   L = 0.5 * sum_r w_r * sum_a (y[r][a] - target[r][a])^2 / sum_r w_r.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -31,9 +32,11 @@ static_assert(kOffB3 + kActions == kP, "parameter layout");
 
 // One thread per row: forward pass and, when T is given, the backward pass
 // down to the per-row layer gradients. Stores the activations and the
-// gradients the parameter reduction needs.
+// gradients the parameter reduction needs. Wr (nullable: all 1) holds the
+// row weights and inv_wsum is 1 / sum_r w_r.
 __global__ __launch_bounds__(256) void mlp_rows_kernel(const float* __restrict__ w, const float* __restrict__ X,
-                                                       const float* __restrict__ T, int n, float* __restrict__ Y,
+                                                       const float* __restrict__ T, const float* __restrict__ Wr,
+                                                       float inv_wsum, int n, float* __restrict__ Y,
                                                        float* __restrict__ H1, float* __restrict__ H2,
                                                        float* __restrict__ D1, float* __restrict__ D2,
                                                        float* __restrict__ DY, float* __restrict__ err) {
@@ -64,15 +67,16 @@ __global__ __launch_bounds__(256) void mlp_rows_kernel(const float* __restrict__
     Y[(size_t)r * kActions + a] = s;
   }
   if (!T) return;
-  const float inv_n = 1.f / (float)n;
+  const float wr = Wr ? Wr[r] : 1.f;
+  const float scale = wr * inv_wsum;
   float dy[kActions], e = 0.f;
   for (int a = 0; a < kActions; ++a) {
     const float d = y[a] - T[(size_t)r * kActions + a];
     e = __fmaf_rn(d, d, e);
-    dy[a] = d * inv_n;
+    dy[a] = d * scale;
     DY[(size_t)r * kActions + a] = dy[a];
   }
-  err[r] = 0.5f * e;
+  err[r] = 0.5f * wr * e;
   float d2[kHidden];
   for (int j = 0; j < kHidden; ++j) {
     float s = 0.f;
@@ -131,8 +135,9 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const float* __restrict__
   grad[p] = s;
 }
 
-// Deterministic sum of the per-row errors (one block).
-__global__ __launch_bounds__(256) void mlp_loss_kernel(const float* __restrict__ err, int n, float* __restrict__ out) {
+// Deterministic sum of the per-row (weighted) errors times inv_wsum (one block).
+__global__ __launch_bounds__(256) void mlp_loss_kernel(const float* __restrict__ err, int n, float inv_wsum,
+                                                       float* __restrict__ out) {
   __shared__ float part[256];
   float s = 0.f;
   for (int r = (int)threadIdx.x; r < n; r += 256) s += err[r];
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(256) void mlp_loss_kernel(const float* __restrict__
     if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = part[0] / (float)n;
+  if (threadIdx.x == 0) out[0] = part[0] * inv_wsum;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction; t is the 1-based step.
@@ -163,9 +168,30 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(float* __restrict__ w, co
 // Device buffers of one training / gradient call.
 struct MlpBuffers {
   float *X = nullptr, *T = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr, *D1 = nullptr, *D2 = nullptr,
-        *DY = nullptr, *err = nullptr, *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
+        *DY = nullptr, *err = nullptr, *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr,
+        *wr = nullptr;
+  float inv_wsum = 1.f;  // 1 / sum of the row weights
   ~MlpBuffers() {
-    for (float* p : {X, T, Y, H1, H2, D1, D2, DY, err, w, g, m, v, loss}) (void)hipFree(p);
+    for (float* p : {X, T, Y, H1, H2, D1, D2, DY, err, w, g, m, v, loss, wr}) (void)hipFree(p);
+  }
+  // Uploads row weights (null: every row weighs 1) and sets inv_wsum.
+  fc_status set_weights(const float* weight, int n, hipStream_t s) {
+    double sum = 0.0;
+    if (weight) {
+      for (int r = 0; r < n; ++r) {
+        if (!(weight[r] >= 0.f) || !std::isfinite(weight[r]))
+          return fc::fail(FC_ERR_INVALID, "scheduler row weight %d is negative or not finite", r);
+        sum += weight[r];
+      }
+      if (!(sum > 0.0)) return fc::fail(FC_ERR_INVALID, "scheduler row weights sum to zero");
+      if (hipMalloc((void**)&wr, sizeof(float) * (size_t)n) != hipSuccess)
+        return fc::fail(FC_ERR_NOMEM, "scheduler buffers: device allocation failed");
+      FC_HIP(hipMemcpyAsync(wr, weight, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, s));
+    } else {
+      sum = (double)n;
+    }
+    inv_wsum = (float)(1.0 / sum);
+    return FC_OK;
   }
   fc_status alloc(int n, bool train) {
     auto A = [](float** p, size_t cnt) { return hipMalloc((void**)p, sizeof(float) * (cnt ? cnt : 1)); };
@@ -184,7 +210,7 @@ struct MlpBuffers {
 static hipError_t launch_rows(const MlpBuffers& b, int n, bool backward, hipStream_t s) {
   const int blocks = (n + 255) / 256;
   hipLaunchKernelGGL(mlp_rows_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, s, b.w, b.X, backward ? b.T : nullptr,
-                     n, b.Y, b.H1, b.H2, b.D1, b.D2, b.DY, b.err);
+                     b.wr, b.inv_wsum, n, b.Y, b.H1, b.H2, b.D1, b.D2, b.DY, b.err);
   return hipGetLastError();
 }
 
@@ -193,7 +219,7 @@ static hipError_t launch_grad(const MlpBuffers& b, int n, hipStream_t s) {
                      b.g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(mlp_loss_kernel, dim3(1), dim3(256), 0, s, b.err, n, b.loss);
+  hipLaunchKernelGGL(mlp_loss_kernel, dim3(1), dim3(256), 0, s, b.err, n, b.inv_wsum, b.loss);
   return hipGetLastError();
 }
 
@@ -281,14 +307,15 @@ extern "C" fc_status fc_sched_forward(fc_graph* g, const float* x, int32_t n, fl
   return FC_OK;
 }
 
-extern "C" fc_status fc_sched_gradient(fc_graph* g, const float* x, const float* target, int32_t n, float* loss_out,
-                                       float* grad_out) {
+extern "C" fc_status fc_sched_gradient(fc_graph* g, const float* x, const float* target, const float* weight,
+                                       int32_t n, float* loss_out, float* grad_out) {
   fc::clear_error();
   fc_status st = check_rows(g, x, n, "fc_sched_gradient");
   if (st != FC_OK) return st;
   if (!target || !grad_out) return fc::fail(FC_ERR_INVALID, "fc_sched_gradient: null argument");
   MlpBuffers b;
   if ((st = b.alloc(n, false)) != FC_OK) return st;
+  if ((st = b.set_weights(weight, n, g->stream)) != FC_OK) return st;
   FC_HIP(hipMemcpyAsync(b.X, x, sizeof(float) * (size_t)n * kFeatures, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b.T, target, sizeof(float) * (size_t)n * kActions, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b.w, g->d_mlp, sizeof(float) * kP, hipMemcpyDeviceToDevice, g->stream));
@@ -316,12 +343,14 @@ static bool row_targets(const fc_trace_row& r, float* tgt) {
 }
 
 extern "C" fc_status fc_sched_train(fc_graph* g, const fc_trace_row* rows, int32_t n_rows, int32_t epochs, float lr,
-                                    uint64_t seed, float* final_loss) {
+                                    uint64_t seed, int32_t weighting, float* final_loss) {
   fc::clear_error();
   if (!g || (n_rows > 0 && !rows)) return fc::fail(FC_ERR_INVALID, "fc_sched_train: null argument");
   if (n_rows <= 0 || epochs <= 0 || !(lr > 0.f))
     return fc::fail(FC_ERR_INVALID, "fc_sched_train: need rows, epochs > 0 and lr > 0");
-  std::vector<float> X, T;
+  if (weighting != FC_SCHED_WEIGHT_UNIFORM && weighting != FC_SCHED_WEIGHT_TIME)
+    return fc::fail(FC_ERR_INVALID, "fc_sched_train: unknown weighting %d", weighting);
+  std::vector<float> X, T, Wt;
   X.reserve((size_t)n_rows * kFeatures);
   T.reserve((size_t)n_rows * kActions);
   int32_t n = 0;
@@ -333,14 +362,22 @@ extern "C" fc_status fc_sched_train(fc_graph* g, const fc_trace_row* rows, int32
       X.push_back(rows[i].features[k]);
     }
     T.insert(T.end(), tg, tg + kActions);
+    Wt.push_back(*std::min_element(rows[i].cost_us, rows[i].cost_us + kActions));
     ++n;
   }
   if (n == 0) return fc::fail(FC_ERR_INVALID, "fc_sched_train: no row has positive finite costs");
+  if (weighting == FC_SCHED_WEIGHT_TIME) {  // each row's best remaining time over the mean of those
+    double mean = 0.0;
+    for (float v : Wt) mean += v;
+    mean /= (double)n;
+    for (float& v : Wt) v = (float)((double)v / mean);
+  }
   std::vector<float> w;
   init_params(seed, w);
   MlpBuffers b;
   fc_status st = b.alloc(n, true);
   if (st != FC_OK) return st;
+  if ((st = b.set_weights(weighting == FC_SCHED_WEIGHT_TIME ? Wt.data() : nullptr, n, g->stream)) != FC_OK) return st;
   FC_HIP(hipMemcpyAsync(b.X, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b.T, T.data(), sizeof(float) * T.size(), hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b.w, w.data(), sizeof(float) * kP, hipMemcpyHostToDevice, g->stream));

@@ -16,6 +16,7 @@ import ctypes as C
 import json
 import os
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -35,6 +36,7 @@ GRAPH_KEEP_IDS = 2
 DIR_FILE, DIR_DIRECTED, DIR_UNDIRECTED = 0, 1, 2
 SCHED_FEATURES, SCHED_HIDDEN, SCHED_ACTIONS = 12, 32, 9
 SCHED_PARAMS = 12 * 32 + 32 + 32 * 32 + 32 + 32 * SCHED_ACTIONS + SCHED_ACTIONS
+SCHED_WEIGHT_UNIFORM, SCHED_WEIGHT_TIME = 0, 1
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -125,11 +127,11 @@ _sig("fc_stcon_batch", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32
 _sig("fc_sched_trace", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(TraceRow),
      C.c_int32, _i32p, _i32p)
 _sig("fc_sched_train", C.c_int, C.c_void_p, C.POINTER(TraceRow), C.c_int32, C.c_int32, C.c_float, C.c_uint64,
-     _f32p)
+     C.c_int32, _f32p)
 _sig("fc_sched_set_params", C.c_int, C.c_void_p, _f32p, C.c_int32)
 _sig("fc_sched_get_params", C.c_int, C.c_void_p, _f32p, C.c_int32)
 _sig("fc_sched_forward", C.c_int, C.c_void_p, _f32p, C.c_int32, _f32p)
-_sig("fc_sched_gradient", C.c_int, C.c_void_p, _f32p, _f32p, C.c_int32, _f32p, _f32p)
+_sig("fc_sched_gradient", C.c_int, C.c_void_p, _f32p, _f32p, _f32p, C.c_int32, _f32p, _f32p)
 
 # Every symbol the C header declares (checked by the CPU tests).
 EXPORTED = [
@@ -445,12 +447,14 @@ class Graph:
         return rows
 
     def sched_train(self, feats: np.ndarray, cost: np.ndarray, epochs: int = 2000, lr: float = 3e-3,
-                    seed: int = 1) -> float:
+                    seed: int = 1, weighting: int = SCHED_WEIGHT_UNIFORM) -> float:
         """Train the scheduler network on the device from traced rows; the
-        trained parameters are set on this graph. Returns the final loss."""
+        trained parameters are set on this graph. weighting is
+        SCHED_WEIGHT_UNIFORM or SCHED_WEIGHT_TIME (rows weighted by their
+        smallest cost over the mean of those). Returns the final loss."""
         rows = self._rows(feats, cost)
         loss = C.c_float()
-        _check(_lib.fc_sched_train(self._h, rows, len(feats), epochs, lr, seed, C.byref(loss)))
+        _check(_lib.fc_sched_train(self._h, rows, len(feats), epochs, lr, seed, weighting, C.byref(loss)))
         return loss.value
 
     def sched_set_params(self, w: np.ndarray):
@@ -468,12 +472,20 @@ class Graph:
         _check(_lib.fc_sched_forward(self._h, x.ctypes.data_as(_f32p), len(x), y.ctypes.data_as(_f32p)))
         return y
 
-    def sched_gradient(self, x: np.ndarray, target: np.ndarray):
+    def sched_gradient(self, x: np.ndarray, target: np.ndarray, weight: Optional[np.ndarray] = None):
+        """Loss 0.5*sum_r w_r*sum_a (y-target)^2 / sum_r w_r and its gradient
+        (weight None: every w_r = 1)."""
         x = np.ascontiguousarray(x, np.float32)
         tg = np.ascontiguousarray(target, np.float32)
+        wp = None
+        if weight is not None:
+            wt = np.ascontiguousarray(weight, np.float32)
+            if wt.shape != (len(x),):
+                raise ValueError("weight must have one entry per row")
+            wp = wt.ctypes.data_as(_f32p)
         g = np.empty(SCHED_PARAMS, np.float32)
         loss = C.c_float()
-        _check(_lib.fc_sched_gradient(self._h, x.ctypes.data_as(_f32p), tg.ctypes.data_as(_f32p), len(x),
+        _check(_lib.fc_sched_gradient(self._h, x.ctypes.data_as(_f32p), tg.ctypes.data_as(_f32p), wp, len(x),
                                       C.byref(loss), g.ctypes.data_as(_f32p)))
         return loss.value, g
 

@@ -17,17 +17,19 @@
    learned schedule itself, on new queries, and the network is refitted on
    both sets of rows (one round of dataset aggregation, so that it also sees
    the states its own choices lead to).
-   The final fit's epoch count and the margin by which the network must
-   beat the heuristic's action are chosen on validation graphs (lowest
-   total regret over a grid of both).
+   The final fit's row weighting (every row alike, or each by the time at
+   stake, its best cost), its epoch count and the margin by which the
+   network must beat the heuristic's action are chosen on validation graphs
+   (lowest total regret over a grid of the three).
 3. Report, on the training rows and on held-out graphs and queries:
    regret (cost of the chosen action over the best action, per traced step)
    of the network, the heuristic and the plain both-sides schedule; and the
    end-to-end queries/s of the plain, heuristic and learned schedules with
    verdicts and distances compared bit for bit.
 
-Writes sched_params.npy (float32, FC_SCHED_PARAMS values) and
-sched_report.json into --out.
+Writes sched_params.npy (float32, FC_SCHED_PARAMS values),
+sched_report.json and the traced rows (sched_rows.npz: training and
+validation features, costs and heuristic actions) into --out.
 """
 import argparse
 import json
@@ -187,26 +189,34 @@ def main():
         VF.append(f), VC.append(c), VH.append(h)
         g.close()
     VF, VC, VH = np.concatenate(VF), np.concatenate(VC), np.concatenate(VH)
+    os.makedirs(args.out, exist_ok=True)
+    np.savez(os.path.join(args.out, "sched_rows.npz"), features=F, cost=Cst, heuristic=Hh,
+             valid_features=VF, valid_cost=VC, valid_heuristic=VH)
     margins = [0.0, 0.02, 0.05, 0.1, 0.15, 0.2, 0.3, 0.5, 1.0]
+    weightings = {"uniform": fc.SCHED_WEIGHT_UNIFORM, "time": fc.SCHED_WEIGHT_TIME}
     best = None
     grid = {}
-    for ep in [int(x) for x in args.epochs_grid.split(",") if x.strip()]:
-        t0 = time.time()
-        loss_ep = g0.sched_train(F, Cst, epochs=ep, lr=args.lr, seed=args.seed)
-        w_ep = g0.sched_get_params()
-        vpred = g0.sched_forward(VF)
-        vreg = {m: regret(VC, pick_actions(vpred, VH, m))[1] for m in margins}
-        m_ep = min(margins, key=lambda m: (round(vreg[m], 4), -m))
-        grid[str(ep)] = {"loss": round(float(loss_ep), 4), "margin": m_ep,
-                         "validation_total_regret_by_margin": {str(m): round(v, 4) for m, v in vreg.items()}}
-        log(f"  fit {ep} epochs on {len(F)} rows: loss {loss_ep:.4f} ({time.time() - t0:.1f}s); validation total "
-            f"regret by margin {json.dumps(grid[str(ep)]['validation_total_regret_by_margin'])}; best margin {m_ep}")
-        key = (round(vreg[m_ep], 4), -m_ep)
-        if best is None or key < best[0]:
-            best = (key, ep, m_ep, w_ep, loss_ep, vreg)
-    _, epochs, margin, w, loss, vreg = best
+    for wname, wcode in weightings.items():
+        for ep in [int(x) for x in args.epochs_grid.split(",") if x.strip()]:
+            t0 = time.time()
+            loss_ep = g0.sched_train(F, Cst, epochs=ep, lr=args.lr, seed=args.seed, weighting=wcode)
+            w_ep = g0.sched_get_params()
+            vpred = g0.sched_forward(VF)
+            vreg = {m: regret(VC, pick_actions(vpred, VH, m))[1] for m in margins}
+            m_ep = min(margins, key=lambda m: (round(vreg[m], 4), -m))
+            key_name = f"{wname}/{ep}"
+            grid[key_name] = {"loss": round(float(loss_ep), 4), "margin": m_ep,
+                              "validation_total_regret_by_margin": {str(m): round(v, 4) for m, v in vreg.items()}}
+            log(f"  fit {key_name} epochs on {len(F)} rows: loss {loss_ep:.4f} ({time.time() - t0:.1f}s); validation "
+                f"total regret by margin {json.dumps(grid[key_name]['validation_total_regret_by_margin'])}; "
+                f"best margin {m_ep}")
+            key = (round(vreg[m_ep], 4), -m_ep)
+            if best is None or key < best[0]:
+                best = (key, wname, ep, m_ep, w_ep, loss_ep, vreg)
+    _, weighting, epochs, margin, w, loss, vreg = best
     g0.sched_set_params(w)
-    log(f"chosen: {epochs} epochs, margin {margin}; heuristic validation regret {regret(VC, VH)[1]:.4f}")
+    log(f"chosen: {weighting} weighting, {epochs} epochs, margin {margin}; heuristic validation regret "
+        f"{regret(VC, VH)[1]:.4f}")
 
     def report_rows(feats, cost, heur):
         pred = g0.sched_forward(feats)
@@ -219,7 +229,8 @@ def main():
         return rows
 
     report = {"train": report_rows(F, Cst, Hh), "validation": report_rows(VF, VC, VH), "held_out": {},
-              "end_to_end": {}, "loss": float(loss), "epochs": epochs, "lr": args.lr, "margin": margin,
+              "end_to_end": {}, "loss": float(loss), "weighting": weighting, "epochs": epochs, "lr": args.lr,
+              "margin": margin,
               "validation_total_regret_by_margin": {str(m): round(v, 4) for m, v in vreg.items()},
               "epochs_grid": grid, "first_fit_epochs": args.epochs, "repeats": args.repeats}
     log("train-set regret " + json.dumps(report["train"]))
@@ -238,7 +249,6 @@ def main():
         g.close()
     g0.close()
 
-    os.makedirs(args.out, exist_ok=True)
     np.save(os.path.join(args.out, "sched_params.npy"), w.astype(np.float32), allow_pickle=False)
     with open(os.path.join(args.out, "sched_report.json"), "w") as fh:
         json.dump(report, fh, indent=1)

@@ -340,17 +340,25 @@ typedef struct fc_trace_row {
  * (nullable) receives the distances. */
 fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy, int32_t every,
                          fc_trace_row* rows, int32_t cap, int32_t* n_rows, int32_t* dist_out);
+/* Row weighting of the training loss: every row alike, or each row by the
+ * time at stake, its smallest cost_us over the actions divided by the mean
+ * of that over the rows (so that the loss follows total, not per-step,
+ * regret). */
+#define FC_SCHED_WEIGHT_UNIFORM 0
+#define FC_SCHED_WEIGHT_TIME 1
 /* Train on the device (hand-written forward/backward/Adam kernels), full
  * batch, from a seeded initialisation. Targets are log(cost_us) minus the
- * row's mean over the actions; the loss is 0.5*sum((y-target)^2)/n_rows. */
+ * row's mean over the actions; the loss is
+ * 0.5*sum_r w_r*sum_a (y-target)^2 / sum_r w_r with the weighting's w_r. */
 fc_status fc_sched_train(fc_graph* g, const fc_trace_row* rows, int32_t n_rows, int32_t epochs,
-                         float lr, uint64_t seed, float* final_loss);
+                         float lr, uint64_t seed, int32_t weighting, float* final_loss);
 fc_status fc_sched_set_params(fc_graph* g, const float* params, int32_t n);
 fc_status fc_sched_get_params(fc_graph* g, float* params, int32_t n);
-/* One forward pass (and optionally the gradient of 0.5*sum((y-target)^2)/n
- * w.r.t. params) on the device; used to check the kernels against a CPU oracle. */
+/* One forward pass, or the loss 0.5*sum_r w_r*sum_a (y-target)^2 / sum_r w_r
+ * and its gradient w.r.t. params (weight nullable: every w_r = 1), on the
+ * device; used to check the kernels against a CPU oracle. */
 fc_status fc_sched_forward(fc_graph* g, const float* x, int32_t n, float* y_out);
-fc_status fc_sched_gradient(fc_graph* g, const float* x, const float* target, int32_t n,
+fc_status fc_sched_gradient(fc_graph* g, const float* x, const float* target, const float* weight, int32_t n,
                             float* loss_out, float* grad_out);
 
 #ifdef __cplusplus

@@ -8,11 +8,15 @@ expands both frontiers top-down, bernaschi/ST_CONN_ONE_ATOMIC/current-st/
 make_stcon.c:960-1200); it is restated here in float64 numpy:
 
   h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), y = W3 h2 + b3
-  L  = 0.5 * sum_r sum_a (y[r,a] - target[r,a])^2 / n
+  L  = 0.5 * sum_r w_r * sum_a (y[r,a] - target[r,a])^2 / sum_r w_r
 
-with the flat parameter layout W1[32][12] b1[32] W2[32][32] b2[32] W3[9][32]
-b3[9]. Pinning: the analytic gradient is checked against central finite
-differences in float64 (tests/test_sched_oracle.py), the forward pass
+(w_r = 1 unless row weights are given; row_weights() gives the
+time-at-stake weighting), with the flat parameter layout W1[32][12] b1[32]
+W2[32][32] b2[32] W3[9][32] b3[9]. Pinning: the analytic gradient, weighted
+and not, is checked against central finite differences in float64
+(tests/test_sched_oracle.py); the weighting against its defining
+identities (integer weights equal repeated rows, a common factor changes
+nothing, a zero weight drops the row); the forward pass
 against hand-built networks with known outputs, and the initialisation and
 Adam step against their published definitions (Glorot & Bengio 2010 uniform
 bound sqrt(6/(fan_in+fan_out)); Kingma & Ba 2015 with bias correction).
@@ -45,20 +49,21 @@ def forward(w, X):
     return h2 @ W3.T + b3
 
 
-def loss_grad(w, X, T):
-    """Loss and gradient (flat, float64) of 0.5*sum((y-T)^2)/n."""
+def loss_grad(w, X, T, weight=None):
+    """Loss and gradient (flat, float64) of
+    0.5*sum_r w_r*sum_a (y-T)^2 / sum_r w_r (weight None: all w_r = 1)."""
     W1, b1, W2, b2, W3, b3 = unpack(w)
     X = np.asarray(X, np.float64)
     T = np.asarray(T, np.float64)
-    n = len(X)
+    wr = np.ones(len(X)) if weight is None else np.asarray(weight, np.float64)
     z1 = X @ W1.T + b1
     h1 = np.maximum(z1, 0.0)
     z2 = h1 @ W2.T + b2
     h2 = np.maximum(z2, 0.0)
     y = h2 @ W3.T + b3
     d = y - T
-    loss = 0.5 * float((d * d).sum()) / n
-    dy = d / n
+    loss = 0.5 * float((wr[:, None] * d * d).sum()) / wr.sum()
+    dy = d * (wr / wr.sum())[:, None]
     gW3 = dy.T @ h2
     gb3 = dy.sum(0)
     d2 = (dy @ W3) * (z2 > 0)
@@ -74,6 +79,13 @@ def row_targets(cost):
     """Training targets of traced rows: log(cost) minus the row mean."""
     lc = np.log(np.asarray(cost, np.float64))
     return lc - lc.mean(axis=1, keepdims=True)
+
+
+def row_weights(cost):
+    """Time-at-stake row weights: each row's smallest cost over the
+    actions divided by the mean of those over the rows."""
+    m = np.asarray(cost, np.float64).min(axis=1)
+    return m / m.mean()
 
 
 def init_params(seed):

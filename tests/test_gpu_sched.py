@@ -44,22 +44,40 @@ def test_forward_and_gradient_match_oracle(fc):
             assert abs(loss - lo) <= 1e-5 * max(1.0, lo)
             scale = np.abs(go).max()
             assert np.abs(grad - go).max() <= 1e-4 * scale, np.abs(grad - go).max() / scale
+            # row-weighted loss (some rows weigh nothing)
+            wr = np.random.default_rng(seed + 10).uniform(0.0, 4.0, len(X)).astype(np.float32)
+            wr[::7] = 0.0
+            say(f"sched weighted gradient seed={seed}")
+            loss, grad = g.sched_gradient(X, T, wr)
+            lo, go = mlp.loss_grad(w, X, T, wr)
+            assert abs(loss - lo) <= 1e-5 * max(1.0, lo)
+            scale = np.abs(go).max()
+            assert np.abs(grad - go).max() <= 1e-4 * scale, np.abs(grad - go).max() / scale
+        # bad weights are rejected
+        with pytest.raises(fc.FastconError):
+            g.sched_gradient(X, T, np.full(len(X), -1.0, np.float32))
+        with pytest.raises(fc.FastconError):
+            g.sched_gradient(X, T, np.zeros(len(X), np.float32))
 
 
-def test_one_training_step_matches_oracle(fc):
+@pytest.mark.parametrize("weighting", ["uniform", "time"])
+def test_one_training_step_matches_oracle(fc, weighting):
     """fc_sched_train with one epoch = seeded init + one Adam step on the
-    full batch; the result equals the oracle's."""
+    full batch; the result equals the oracle's, with rows weighted alike or
+    by the time at stake."""
     hg = fc.gen_grid(8, 8, 0.0, 1)
     rng = np.random.default_rng(4)
     X = rng.uniform(0, 1, (200, mlp.F)).astype(np.float32)
     cost = np.exp(rng.normal(3, 1, (200, mlp.A))).astype(np.float32)
     T = mlp.row_targets(cost)
+    wr = mlp.row_weights(cost) if weighting == "time" else None
+    code = fc.SCHED_WEIGHT_TIME if weighting == "time" else fc.SCHED_WEIGHT_UNIFORM
     w0 = mlp.init_params(11).astype(np.float64)
-    _, g0 = mlp.loss_grad(w0, X, T)
+    l0, g0 = mlp.loss_grad(w0, X, T, wr)
     w1, _, _ = mlp.adam_step(w0, g0, np.zeros(mlp.P), np.zeros(mlp.P), 1, 1e-3)
     with fc.Graph(hg) as g:
-        say("sched train one epoch")
-        g.sched_train(X, cost, epochs=1, lr=1e-3, seed=11)
+        say(f"sched train one epoch weighting={weighting}")
+        g.sched_train(X, cost, epochs=1, lr=1e-3, seed=11, weighting=code)
         w = g.sched_get_params()
     # Adam's first step is lr*sign(g): compare wherever g is not so small
     # that float32 rounding could flip its sign
@@ -67,6 +85,9 @@ def test_one_training_step_matches_oracle(fc):
     close = np.abs(w - w1) <= 2e-6
     assert close[big].all()
     assert close.mean() > 0.95
+    if weighting == "time":  # the weighting moved the step somewhere
+        _, gu = mlp.loss_grad(w0, X, T)
+        assert (np.sign(gu[big]) != np.sign(g0[big])).any()
 
 
 def test_training_reduces_loss(fc):

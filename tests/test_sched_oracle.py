@@ -1,6 +1,7 @@
-"""Pins the scheduler-network oracle on the CPU: analytic gradient against
-central finite differences, forward pass against hand-built networks, and
-the initialisation and Adam step against their definitions."""
+"""Pins the scheduler-network oracle on the CPU: analytic gradient (with and
+without row weights) against central finite differences, the row weighting
+against its defining identities, forward pass against hand-built networks,
+and the initialisation and Adam step against their definitions."""
 import numpy as np
 
 from oracle import mlp
@@ -29,6 +30,42 @@ def test_gradient_matches_finite_differences():
     # the loss is the defined mean squared error
     y = mlp.forward(w, X)
     assert np.isclose(loss, 0.5 * ((y - T) ** 2).sum() / len(X))
+
+
+def test_weighted_gradient_matches_finite_differences():
+    w, X, T = _rand(4, n=9)
+    wr = np.random.default_rng(1).uniform(0.1, 3.0, len(X))
+    loss, g = mlp.loss_grad(w, X, T, wr)
+    h = 1e-6
+    idx = np.concatenate([np.arange(0, mlp.P, 41), [mlp.P - 1]])
+    for i in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (mlp.loss_grad(wp, X, T, wr)[0] - mlp.loss_grad(wm, X, T, wr)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_row_weight_identities():
+    """Integer weights equal repeated rows; a common factor changes nothing;
+    a zero weight drops its row; all-ones weights are the unweighted loss."""
+    w, X, T = _rand(6, n=5)
+    k = np.array([1, 3, 0, 2, 1])
+    lw, gw = mlp.loss_grad(w, X, T, k)
+    lr_, gr = mlp.loss_grad(w, np.repeat(X, k, axis=0), np.repeat(T, k, axis=0))
+    assert np.isclose(lw, lr_) and np.allclose(gw, gr, rtol=1e-12, atol=1e-14)
+    l7, g7 = mlp.loss_grad(w, X, T, 7.0 * k)
+    assert np.isclose(lw, l7) and np.allclose(gw, g7, rtol=1e-12, atol=1e-14)
+    l1, g1 = mlp.loss_grad(w, X, T, np.ones(len(X)))
+    l0, g0 = mlp.loss_grad(w, X, T)
+    assert np.isclose(l1, l0) and np.allclose(g1, g0, rtol=1e-12, atol=1e-14)
+
+
+def test_row_weights_are_time_at_stake():
+    cost = np.array([[4.0, 2.0, 9.0], [10.0, 30.0, 20.0], [6.0, 6.0, 6.0]])
+    # smallest costs 2, 10, 6 with mean 6
+    assert np.allclose(mlp.row_weights(cost), [2 / 6, 10 / 6, 1.0])
+    assert np.isclose(mlp.row_weights(cost).mean(), 1.0)
 
 
 def test_planted_mistakes_are_caught():
