@@ -205,10 +205,26 @@ def main():
         o = 28 if g.offsets64 else 20
         b_bfs = 8 * e_tot + o * v_tot + 4 * V * len(bfs_roots)
         bw = b_bfs / (ms_tot / 1e3) / 1e9
+        # Measured HBM bytes per BFS (FETCH_SIZE and WRITE_SIZE counter runs of
+        # the phase launches and the un-permutation, scaled by a copy kernel of
+        # known size), kept in profiles/ for the workload they were measured on.
+        # They fall below the algorithmic bytes because bottom-up levels stop
+        # scanning a vertex's list at its first frontier neighbour, while the
+        # algorithmic count charges every edge.
+        b_traffic, b_src = None, None
+        bpmc_path = os.path.join(ROOT, "profiles", "phase_kernel_bfs_rmat_s22_pmc.json")
+        if os.path.exists(bpmc_path):
+            bp = json.load(open(bpmc_path))
+            m = bp.get("match", {})
+            if (m.get("scale"), m.get("edgefactor"), m.get("roots"), m.get("seed")) == \
+                    (args.scale, args.edgefactor, args.roots, args.seed):
+                b_traffic, b_src = int(bp["traffic_bytes_per_bfs"]), bpmc_path[len(ROOT) + 1:]
         bfs = {"roots": len(bfs_roots), "gteps": round(gteps, 2), "gteps_graph500": round(gteps / 2, 2),
                "avg_ms": round(ms_tot / len(bfs_roots), 4), "avg_levels": float(np.mean(lv)),
                "roofline": {"bound": "hbm", "achieved": round(bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(bw / HBM_PEAK_GBS, 4), "kernel": "fcd::phase_kernel<int> (BFS)"}}
+                            "frac": round(bw / HBM_PEAK_GBS, 4), "kernel": "fcd::phase_kernel<int> (BFS)",
+                            "bytes_per_bfs": int(b_bfs / len(bfs_roots)), "traffic_per_bfs": b_traffic,
+                            "traffic_source": b_src}}
         log(f"bfs: {gteps:.1f} GTEPS over {len(bfs_roots)} roots, {ms_tot / len(bfs_roots):.3f} ms avg, "
             f"{bw:.0f} GB/s algorithmic = {bw / HBM_PEAK_GBS:.1%} of HBM peak")
 
