@@ -1255,7 +1255,7 @@ struct BatchWorkspace {
   int graph_force = -2;
   int tuned_vw = -1;       // autotuner result (0 = degree-chosen)
   int tuned_tile = -1;
-  double est_launches = 12.0;
+  double est_launches = 12.0;  // working launches per wave (running estimate)
   int32_t* h_dist = nullptr;  // pinned
   int32_t* h_phase = nullptr; // pinned
   char* snap = nullptr;        // tracing: copy of the workspace
@@ -1511,7 +1511,9 @@ static fc_status batch_drive(fc_graph* g, LaneRun* lr, int nl, bool use_graph, f
   FC_HIP(hipEventRecord(g->ev0, b0->stream));
   for (int k = 1; k < nl; ++k) FC_HIP(hipStreamWaitEvent(lr[k].b->stream, g->ev0, 0));
   for (int k = 0; k < nl; ++k) {
-    lr[k].expected = (int64_t)std::ceil((double)lr[k].nw * lr[k].b->est_launches) + 2;
+    // 10% above the expected working launches: a lane that runs out waits
+    // for the host's next poll, which costs more than a few no-op launches
+    lr[k].expected = (int64_t)std::ceil((double)lr[k].nw * lr[k].b->est_launches * 1.1) + 2;
     lr[k].launched = 0;
     lr[k].round = 0;
     lr[k].done = lr[k].nw == 0;
@@ -1558,11 +1560,7 @@ static fc_status batch_drive(fc_graph* g, LaneRun* lr, int nl, bool use_graph, f
   FC_HIP(hipEventSynchronize(g->ev1));
   FC_HIP(hipEventElapsedTime(ms, g->ev0, g->ev1));
   int64_t total = 0;
-  for (int k = 0; k < nl; ++k) {
-    if (lr[k].nw > 0)
-      lr[k].b->est_launches = 0.5 * lr[k].b->est_launches + 0.5 * ((double)lr[k].launched / (double)lr[k].nw);
-    total += lr[k].launched;
-  }
+  for (int k = 0; k < nl; ++k) total += lr[k].launched;
   if (launches_out) *launches_out = total;
   return FC_OK;
 }
@@ -1605,6 +1603,12 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
   uint32_t err = 0;
   for (int k = 0; k < nl; ++k) err |= cs[k].err;
   if (err) return fail(FC_ERR_CAPACITY, "batch kernel: list guard tripped (err=%u)", err);
+  // Running estimate of the working launches (finalised phases) per wave.
+  // It follows the phases, not the launches issued: those are rounded up to
+  // whole captured graphs, and an estimate fed by them would only grow.
+  for (int k = 0; k < nl; ++k)
+    if (lr[k].nw > 0)
+      lr[k].b->est_launches = 0.5 * lr[k].b->est_launches + 0.5 * ((double)cs[k].phases / (double)lr[k].nw);
   if (ctl_out) {
     *ctl_out = cs[0];
     for (int k = 1; k < nl; ++k) {

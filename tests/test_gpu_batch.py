@@ -148,3 +148,25 @@ def test_batch_lanes_agree(fc, orc):
                 assert np.array_equal(res.dist, exp[:n]), (lanes, n)
         with pytest.raises(fc.FastconError):
             g.stcon_batch(s, t, lanes=5)
+
+
+def test_batch_launch_surplus_stays_bounded(fc, orc):
+    """Launches are issued in captured graphs of 16 against a running
+    estimate of the working launches per wave; over many calls the surplus
+    (no-op launches after the last phase) stays within one graph and a
+    margin per lane instead of growing call by call."""
+    hg = fc.gen_rmat(14, 16, seed=43)
+    r = hg.sample_roots(1024, 44)
+    s, t = r[:512], r[512:]
+    exp = _oracle_dist(orc, hg, s, t)
+    with fc.Graph(hg) as g:
+        surplus = []
+        for k in range(60):
+            if k % 20 == 0:
+                say(f"batch launch surplus call {k}")
+            res = g.stcon_batch(s, t, lanes=3)
+            assert np.array_equal(res.dist, exp)
+            surplus.append(res.stats["launches"] - res.stats["phases"])
+        lanes = 3
+        assert max(surplus[-20:]) <= lanes * 24, surplus
+        assert np.mean(surplus[-20:]) <= np.mean(surplus[5:25]) + 16, surplus
