@@ -269,8 +269,13 @@ __device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bo
 }
 
 // Edge-parallel expansion of the hub queue: the concatenated adjacency of all
-// hub vertices is cut into chunks of kHubChunk edges handed out across waves.
-constexpr int kHubChunk = 256;
+// hub vertices is cut into chunks handed out across waves. A chunk is 256
+// edges (four passes of one edge per lane) when there are enough edges for
+// every wave to get a chunk of that size, else 128 or 64: when the hub edges
+// are few (the second level of a visit from one root), a wave's chain of
+// dependent loads per pass, not the edge count, sets the level's time, and
+// smaller chunks spread the passes over more waves.
+constexpr int kHubChunkMax = 256;
 
 template <typename OffT>
 __device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par, uint64_t hubpk, int gwave,
@@ -283,7 +288,9 @@ __device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool
   const int lane = lane_id();
   const int32_t* hq = p.hq[a.c][cur_par];
   const int64_t* hb = p.hb[a.c][cur_par];
-  for (uint64_t cb = (uint64_t)gwave * kHubChunk; cb < H; cb += (uint64_t)nwaves * kHubChunk) {
+  const uint64_t nw = (uint64_t)nwaves;
+  const int chunk = H >= nw * kHubChunkMax ? kHubChunkMax : H >= nw * (kHubChunkMax / 2) ? kHubChunkMax / 2 : kWave;
+  for (uint64_t cb = (uint64_t)gwave * (uint64_t)chunk; cb < H; cb += nw * (uint64_t)chunk) {
     // Last slot with hb[slot] <= cb (hb[0] == 0): a 64-ary search, each lane
     // testing one of 64 evenly spaced probes per round.
     uint32_t lo = 0, hi = nh;
@@ -298,7 +305,7 @@ __device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool
       const uint32_t nhi = lo + step;
       hi = nhi < hi ? nhi : hi;
     }
-    for (int k = 0; k < kHubChunk; k += kWave) {
+    for (int k = 0; k < chunk; k += kWave) {
       const uint64_t g = cb + (uint64_t)k + (uint64_t)lane;
       const bool act = g < H;
       int32_t v = -1, u = 0;
