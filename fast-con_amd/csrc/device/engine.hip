@@ -152,8 +152,11 @@ fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const 
   // caller's arrays can go away, and before timing starts.
   FC_HIP(hipStreamSynchronize(g->stream));
 
+  // Launch budget: the running estimate of level phases per query plus the
+  // reset phase. A single BFS needs no extra margin: a short budget costs one
+  // more host poll, and each surplus launch of the full grid about 4 us.
   const double est = g->est_levels[mode];
-  int64_t expected = (int64_t)std::ceil((double)n * (est + 1.0)) + 1;
+  int64_t expected = (int64_t)std::ceil((double)n * (est + 1.0));
   int64_t launched = 0;
   FC_HIP(hipEventRecord(g->ev0, g->stream));
   for (int round = 0;; ++round) {
