@@ -96,6 +96,7 @@ static VisitParams<OffT> make_params(fc_graph* g, int force_action) {
   p.qt = g->d_qt;
   p.out = g->d_out;
   p.force_action = force_action;
+  p.lazy_state = g->relabeled ? 1 : 0;
   return p;
 }
 
@@ -393,15 +394,19 @@ static fc_status check_out(const fcd::VisitOut& o) {
 
 // Levels (and parents) of a BFS on a relabelled graph back in caller ids:
 // out[v] = lv[perm[v]], parents mapped through iperm. One coalesced pass.
+// Levels and parents back in caller ids. A BFS writes state and parent only
+// for the vertices it reaches; the visited bitmap says which those are.
 __global__ void unpermute_kernel(const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
-                                 const int32_t* __restrict__ lv, const int32_t* __restrict__ par,
-                                 int32_t* __restrict__ lv_out, int32_t* __restrict__ par_out, int64_t V) {
+                                 const uint32_t* __restrict__ reached, const int32_t* __restrict__ lv,
+                                 const int32_t* __restrict__ par, int32_t* __restrict__ lv_out,
+                                 int32_t* __restrict__ par_out, int64_t V) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
     const int32_t k = perm[v];
-    lv_out[v] = lv[k];
+    const bool r = (reached[(uint32_t)k >> 5] >> (k & 31)) & 1u;
+    lv_out[v] = r ? lv[k] : -1;
     if (par) {
-      const int32_t q = par[k];
+      const int32_t q = r ? par[k] : -1;
       par_out[v] = (uint32_t)q < (uint32_t)V ? iperm[q] : q;
     }
   }
@@ -428,8 +433,8 @@ extern "C" fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32
     if (par && !g->d_par_out) FC_HIP(hipMalloc((void**)&g->d_par_out, bytes));
     FC_HIP(hipEventRecord(g->ev0, g->stream));
     const int blocks = (int)std::min<int64_t>((g->V + 255) / 256, (int64_t)g->cus * 8);
-    hipLaunchKernelGGL(unpermute_kernel, dim3(blocks), dim3(256), 0, g->stream, g->d_perm, g->d_iperm, g->d_state,
-                       par ? g->d_parent : nullptr, g->d_lv_out, g->d_par_out, (int64_t)g->V);
+    hipLaunchKernelGGL(unpermute_kernel, dim3(blocks), dim3(256), 0, g->stream, g->d_perm, g->d_iperm, g->d_visited,
+                       g->d_state, par ? g->d_parent : nullptr, g->d_lv_out, g->d_par_out, (int64_t)g->V);
     FC_HIP(hipGetLastError());
     FC_HIP(hipEventRecord(g->ev1, g->stream));
     FC_HIP(hipEventSynchronize(g->ev1));

@@ -671,14 +671,21 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
   Decision d = ctl->dec;
   if (phase == kPhaseReset) {
     const int32_t src = ctl->src, dst = st ? ctl->dst : -1;
-    for (int64_t v = gtid; v < p.V; v += nthr) {
-      int32_t s = -1;
-      if (v == src)
-        s = 0;
-      else if (v == dst)
-        s = kColorBit;
-      p.state[v] = s;
-      if (parent) parent[v] = s != -1 ? (int32_t)v : -1;
+    if (st || !p.lazy_state) {
+      for (int64_t v = gtid; v < p.V; v += nthr) {
+        int32_t s = -1;
+        if (v == src)
+          s = 0;
+        else if (v == dst)
+          s = kColorBit;
+        p.state[v] = s;
+        if (parent) parent[v] = s != -1 ? (int32_t)v : -1;
+      }
+    } else if (gtid == 0) {
+      // BFS on a renumbered graph: only reached vertices' state and parent
+      // are written (the visited bitmap marks them), so only the root's here
+      p.state[src] = 0;
+      if (parent) parent[src] = src;
     }
     for (int64_t w = bitmaps ? gtid : (int64_t)nwords; w < nwords; w += nthr) {
       const uint32_t bs = ((uint32_t)src >> 5) == (uint32_t)w ? 1u << (src & 31) : 0u;
@@ -687,9 +694,11 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
       p.fbm[0][0][w] = bs;
       p.fbm[0][1][w] = 0u;
       p.fbm[0][2][w] = 0u;
-      p.fbm[1][0][w] = bt;
-      p.fbm[1][1][w] = 0u;
-      p.fbm[1][2][w] = 0u;
+      if (st) {  // the second colour's frontier bitmaps serve s–t modes only
+        p.fbm[1][0][w] = bt;
+        p.fbm[1][1][w] = 0u;
+        p.fbm[1][2][w] = 0u;
+      }
     }
   } else {
     if (p.force_action >= 0) apply_action(d, p.force_action);

@@ -111,7 +111,8 @@ struct VisitParams {
   const int32_t* qt;        // query targets
   VisitOut* out;            // results (device), nq entries
   int32_t force_action;     // -1, or a scheduler action applied to this launch (tracing)
-  int32_t pad;
+  int32_t lazy_state;       // BFS leaves unreached state/parent unwritten (renumbered graphs:
+                            // the un-permutation takes reached-ness from the visited bitmap)
 };
 
 // Scheduler features of a pending step, shared by the kernel and the host tracer.
