@@ -6,7 +6,8 @@ learned schedules (the learned one with random network weights and no
 margin, so that it takes arbitrary actions, bottom-up and wide steps
 included) and the helper-seed verdict are compared with the host BFS. Half
 of the graphs are built with a low hub threshold so that the hub queues and
-hub lists carry ordinary vertices too."""
+hub lists carry ordinary vertices too; every third graph is uploaded with
+64-bit offsets and every fourth keeps the caller's vertex numbering."""
 import sys
 
 import numpy as np
@@ -105,9 +106,11 @@ def test_random_family_matches_host_bfs(fc, orc, monkeypatch, seed):
         s, t = _pairs(rng, hg, 192)
         exp = np.array([orc.stcon(hg.offsets, hg.columns, a, b)[1] for a, b in zip(s, t)], np.int32)
         roots = rng.integers(0, hg.V, 3)
-        with fc.Graph(hg) as g:
+        off64, keep = k % 3 == 2, k % 4 == 3
+        with fc.Graph(hg, force64=off64, keep_ids=keep) as g:
             for r in roots:
-                say(f"random seed={seed} {name} V={hg.V} E={hg.E} low_hub={low_hub} bfs root={r}")
+                say(f"random seed={seed} {name} V={hg.V} E={hg.E} low_hub={low_hub} off64={off64} keep_ids={keep} "
+                    f"bfs root={r}")
                 lv, par, _ = g.bfs(int(r), levels=True, parents=True)
                 assert np.array_equal(lv, orc.bfs(hg.offsets, hg.columns, int(r))[0]), name
                 assert orc.validate_bfs_tree(hg.offsets, hg.columns, int(r), lv, par) == 0, name
