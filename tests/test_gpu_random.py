@@ -7,7 +7,9 @@ margin, so that it takes arbitrary actions, bottom-up and wide steps
 included) and the helper-seed verdict are compared with the host BFS. Half
 of the graphs are built with a low hub threshold so that the hub queues and
 hub lists carry ordinary vertices too; every third graph is uploaded with
-64-bit offsets and every fourth keeps the caller's vertex numbering."""
+64-bit offsets and every fourth keeps the caller's vertex numbering. The
+plain visit's virtual-warp width is fixed to 4, 8, 16, 32 or 64 on every
+other graph, and the grid varies between 1 and 4 blocks per CU."""
 import sys
 
 import numpy as np
@@ -103,6 +105,12 @@ def test_random_family_matches_host_bfs(fc, orc, monkeypatch, seed):
             monkeypatch.setenv("FASTCON_HUB_THRESHOLD", "6")
         else:
             monkeypatch.delenv("FASTCON_HUB_THRESHOLD", raising=False)
+        vw = (4, 8, 16, 32, 64)[(k // 2) % 5] if k % 2 == 0 else 0
+        if vw:
+            monkeypatch.setenv("FASTCON_VW", str(vw))
+        else:
+            monkeypatch.delenv("FASTCON_VW", raising=False)
+        monkeypatch.setenv("FASTCON_BLOCKS_PER_CU", str(1 + k % 4))
         s, t = _pairs(rng, hg, 192)
         exp = np.array([orc.stcon(hg.offsets, hg.columns, a, b)[1] for a, b in zip(s, t)], np.int32)
         roots = rng.integers(0, hg.V, 3)
@@ -110,7 +118,7 @@ def test_random_family_matches_host_bfs(fc, orc, monkeypatch, seed):
         with fc.Graph(hg, force64=off64, keep_ids=keep) as g:
             for r in roots:
                 say(f"random seed={seed} {name} V={hg.V} E={hg.E} low_hub={low_hub} off64={off64} keep_ids={keep} "
-                    f"bfs root={r}")
+                    f"vw={vw} bpc={1 + k % 4} bfs root={r}")
                 lv, par, _ = g.bfs(int(r), levels=True, parents=True)
                 assert np.array_equal(lv, orc.bfs(hg.offsets, hg.columns, int(r))[0]), name
                 assert orc.validate_bfs_tree(hg.offsets, hg.columns, int(r), lv, par) == 0, name
