@@ -9,7 +9,9 @@ of the graphs are built with a low hub threshold so that the hub queues and
 hub lists carry ordinary vertices too; every third graph is uploaded with
 64-bit offsets and every fourth keeps the caller's vertex numbering. The
 plain visit's virtual-warp width is fixed to 4, 8, 16, 32 or 64 on every
-other graph, and the grid varies between 1 and 4 blocks per CU."""
+other graph, and the grid varies between 1 and 4 blocks per CU. One more
+batched run per graph draws its width, hub chunk, lane count, launch mode
+and blocks per CU at random."""
 import sys
 
 import numpy as np
@@ -111,6 +113,7 @@ def test_random_family_matches_host_bfs(fc, orc, monkeypatch, seed):
         else:
             monkeypatch.delenv("FASTCON_VW", raising=False)
         monkeypatch.setenv("FASTCON_BLOCKS_PER_CU", str(1 + k % 4))
+        monkeypatch.setenv("FASTCON_BATCH_BLOCKS_PER_CU", str(1 + k % 3))
         s, t = _pairs(rng, hg, 192)
         exp = np.array([orc.stcon(hg.offsets, hg.columns, a, b)[1] for a, b in zip(s, t)], np.int32)
         roots = rng.integers(0, hg.V, 3)
@@ -129,6 +132,15 @@ def test_random_family_matches_host_bfs(fc, orc, monkeypatch, seed):
                 res = g.stcon_batch(s, t, policy=policy)
                 bad = np.flatnonzero(res.dist != exp)
                 assert bad.size == 0, (name, policy, bad[:5], res.dist[bad[:5]], exp[bad[:5]])
+            bvw = int(rng.choice([0, 4, 8, 16, 32, 64]))
+            btile = int(rng.choice([64, 128, 256, 512, 1024, 4096]))
+            blanes = int(rng.integers(1, 5))
+            bgraph = int(rng.integers(0, 2))
+            say(f"random seed={seed} {name} batch vw={bvw} tile={btile} lanes={blanes} graph={bgraph} "
+                f"bpc={1 + k % 3}")
+            res = g.stcon_batch(s, t, policy=fc.POLICY_HEURISTIC, vw=bvw, tile=btile, use_graph=bgraph, lanes=blanes)
+            bad = np.flatnonzero(res.dist != exp)
+            assert bad.size == 0, (name, bvw, btile, blanes, bgraph, bad[:5], res.dist[bad[:5]], exp[bad[:5]])
             w = rng.normal(0, 0.5, mlp.P).astype(np.float32)
             g.sched_set_params(w)
             for lanes in (1, 3):
