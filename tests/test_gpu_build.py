@@ -51,3 +51,28 @@ def test_gen_rmat_gpu_equals_host(fc, scale, ef, a, seed):
     t2 = time.time()
     say(f"rmat s{scale} ef{ef}: V={host.V} E={host.E}; host {t1 - t0:.3f}s, gpu {t2 - t1:.3f}s")
     assert _same(host, gpu)
+
+
+def test_csr_from_edges_gpu_edge_shapes(fc, orc):
+    """Shapes the radix build must get right: nothing but self-loops and
+    dropped pairs, one edge repeated many times, a star whose hub has 200 K
+    neighbours, ids at the top of the range, and vertex counts on either side
+    of a power of two (the key width changes there)."""
+    rng = np.random.default_rng(9)
+    cases = {
+        "self-loops and dropped pairs only": (50, np.array([[i, i] for i in range(50)] + [[-1, -1]] * 10)),
+        "one edge repeated": (10, np.array([[3, 7]] * 5000)),
+        "star of 200K leaves": (200001, np.stack([np.zeros(200000, np.int64), np.arange(1, 200001)], 1)),
+        "top ids": (1000, np.array([[999, 998], [998, 999], [999, 0], [0, 999], [997, 999]])),
+    }
+    for V in ((1 << 16), (1 << 16) + 1, (1 << 20) - 1, 1 << 20):
+        e = rng.integers(0, V, size=(4096, 2))
+        e[:16, 0] = V - 1
+        cases[f"V={V}"] = (V, e)
+    for name, (V, e) in cases.items():
+        say(f"edge shape: {name}")
+        host = fc.csr_from_edges(e, V)
+        gpu = fc.csr_from_edges_gpu(e, V)
+        assert _same(host, gpu), name
+        off, col = orc.canonical(e, V)
+        assert np.array_equal(gpu.offsets, off) and np.array_equal(gpu.columns, col), name
