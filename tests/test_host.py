@@ -246,3 +246,30 @@ def test_validate_bfs_depth_gap(fc, orc):
     assert ok and d.tolist() == [0, 1, 1, 2]
     ok, msg, _ = hg.validate_bfs(0, np.array([0, 0, 1, 2], np.int32))
     assert not ok and "too far apart" in msg
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus above 1 exits with a message before
+    touching a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_python_constants_match_the_header(fc):
+    """The ctypes mirror's constants equal the C header's #defines."""
+    hdr = open(os.path.join(ROOT, "include", "fastcon.h")).read()
+    defs = dict(re.findall(r"#define\s+(FC_[A-Z0-9_]+)\s+(-?\d+)u?\b", hdr))
+    pairs = {"FC_SCHED_FEATURES": fc.SCHED_FEATURES, "FC_SCHED_HIDDEN": fc.SCHED_HIDDEN,
+             "FC_SCHED_ACTIONS": fc.SCHED_ACTIONS, "FC_SCHED_WEIGHT_UNIFORM": fc.SCHED_WEIGHT_UNIFORM,
+             "FC_SCHED_WEIGHT_TIME": fc.SCHED_WEIGHT_TIME, "FC_GRAPH_KEEP_ORDER": fc.GRAPH_KEEP_ORDER,
+             "FC_GRAPH_KEEP_IDS": fc.GRAPH_KEEP_IDS}
+    for name, val in pairs.items():
+        assert name in defs, name
+        assert int(defs[name]) == val, (name, defs[name], val)
+    F, H, A = fc.SCHED_FEATURES, fc.SCHED_HIDDEN, fc.SCHED_ACTIONS
+    assert fc.SCHED_PARAMS == F * H + H + H * H + H + H * A + A
