@@ -268,9 +268,11 @@ typedef enum fc_policy {
 typedef struct fc_batch_opts {
   int32_t policy;        /* fc_policy */
   int32_t vw;            /* lanes per vertex for top-down expansion (4..64); 0 = autotuned */
-  int32_t tile;          /* tile schedule (work items per wave-iteration); 0 = autotuned */
-  int32_t use_graph;     /* 1: replay a captured hipGraph per wave; 0: plain launches */
-  int32_t words;         /* 64-bit query words per vertex (1 = 64 queries per wave) */
+  int32_t tile;          /* hub chunk: edges of the hub lists one wave takes at a time, a
+                            multiple of 64 in [64, 65536]; 0 = autotuned */
+  int32_t use_graph;     /* 1: replay captured hipGraphs of 16 phase launches; 0: plain launches */
+  int32_t words;         /* 64-bit query words per vertex: 1 (64 queries per wave) or 0 (the
+                            same); other values are FC_ERR_UNSUPPORTED */
   float   learn_margin;  /* learned policy: the network overrides the heuristic's action only
                             when it predicts a log-cost lower by at least this; < 0 = 0.1 */
   int32_t lanes;         /* waves run concurrently, each on its own stream and workspace
