@@ -545,7 +545,8 @@ __device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& a
   const unsigned long long* FT = p.F[1][ctl->cur[1]];
   const int lane = lane_id();
   const int64_t nthr = (int64_t)nwaves * kWave;
-  for (int64_t base = (int64_t)gwave * kWave; base < p.V; base += nthr) {
+  // isolated vertices (the tail of a degree-ordered numbering) gain no bit
+  for (int64_t base = (int64_t)gwave * kWave; base < p.v_active; base += nthr) {
     const int32_t u = (int32_t)(base + lane);
     unsigned long long accS = 0, accT = 0, needS = 0, needT = 0, vS = 0, vT = 0;
     OffT b = 0, e = 0;
@@ -1438,6 +1439,7 @@ static BatchParams<OffT> batch_params(fc_graph* g, BatchWorkspace* b) {
   p.qt = b->qt;
   p.dist = b->dist;
   p.perm = g->relabeled ? g->d_perm : nullptr;
+  p.v_active = (int32_t)g->v_active;
   p.mlp = g->d_mlp + kMlpParams;  // the copy in mlp_t_index order
   return p;
 }

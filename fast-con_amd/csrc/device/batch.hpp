@@ -122,7 +122,8 @@ struct BatchParams {
   const int32_t* perm;         // caller id -> device id (null: the same)
   int32_t* dist;               // results (device), nq entries
   const float* mlp;
-  int32_t pad[2];
+  int32_t v_active;            // vertices [0, v_active) have degree > 0 (the rest are isolated)
+  int32_t pad;
 };
 
 template <typename OffT>

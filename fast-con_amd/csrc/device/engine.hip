@@ -97,6 +97,7 @@ static VisitParams<OffT> make_params(fc_graph* g, int force_action) {
   p.out = g->d_out;
   p.force_action = force_action;
   p.lazy_state = g->relabeled ? 1 : 0;
+  p.v_active = (int32_t)g->v_active;
   return p;
 }
 
@@ -272,7 +273,15 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
     CK(hipMemcpy(g->d_iperm, g->h_iperm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice));
     CK(hipMemcpy(g->d_off, noff.data(), sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
     if (E > 0) CK(hipMemcpy(g->d_col, ncol.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+    // decreasing degree: the vertices of degree > 0 are a prefix
+    int64_t va = V;
+    while (va > 0 && noff[(size_t)va] == noff[(size_t)va - 1]) --va;
+    for (int64_t v = 0; v < va; ++v)
+      if (noff[(size_t)v + 1] == noff[(size_t)v])
+        return bail(fail(FC_ERR_INVALID, "internal: degree order broken at vertex %lld", (long long)v));
+    g->v_active = va;
   } else {
+    g->v_active = V;
     CK(hipMemcpy(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
   }
   if (E > 0 && !g->relabeled) {

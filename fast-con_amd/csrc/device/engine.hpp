@@ -60,6 +60,8 @@ struct fc_graph {
   // Degree relabelling (absent with FC_GRAPH_KEEP_IDS): the device graph
   // uses new ids; perm maps a caller id to its device id, iperm back.
   bool relabeled = false;
+  int64_t v_active = 0;  // vertices [0, v_active) have degree > 0: the degree-ordered
+                         // numbering puts every isolated vertex after them; V otherwise
   int32_t* d_perm = nullptr;
   int32_t* d_iperm = nullptr;
   std::vector<int32_t> h_perm, h_iperm;

@@ -375,7 +375,9 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool
   const int c = a.c;
   uint32_t* nfbm = a.nfbm;
   const int lane = lane_id();
-  const uint32_t nw64 = ((uint32_t)p.V + 63u) >> 6;
+  // Isolated vertices (the tail of a degree-ordered numbering) can have no
+  // parent: their chunks are skipped (43% of the vertices of R-MAT s22).
+  const uint32_t nw64 = ((uint32_t)p.v_active + 63u) >> 6;
   const int32_t mine = st ? ((c ? kColorBit : 0) | (lvl + 1)) : lvl + 1;
   uint32_t w = (uint32_t)gwave;
   if (w >= nw64) return;

@@ -113,6 +113,8 @@ struct VisitParams {
   int32_t force_action;     // -1, or a scheduler action applied to this launch (tracing)
   int32_t lazy_state;       // BFS leaves unreached state/parent unwritten (renumbered graphs:
                             // the un-permutation takes reached-ness from the visited bitmap)
+  int32_t v_active;         // vertices [0, v_active) have degree > 0 (the rest are isolated)
+  int32_t pad;
 };
 
 // Scheduler features of a pending step, shared by the kernel and the host tracer.
