@@ -70,7 +70,7 @@ fc_status ensure_queries(fc_graph* g, int32_t n) {
 }
 
 template <typename OffT>
-static VisitParams<OffT> make_params(fc_graph* g, int force_action) {
+static VisitParams<OffT> make_params(fc_graph* g) {
   VisitParams<OffT> p;
   std::memset(&p, 0, sizeof(p));
   p.off = (const OffT*)g->d_off;
@@ -95,28 +95,21 @@ static VisitParams<OffT> make_params(fc_graph* g, int force_action) {
   p.qs = g->d_qs;
   p.qt = g->d_qt;
   p.out = g->d_out;
-  p.force_action = force_action;
   p.lazy_state = g->relabeled ? 1 : 0;
   p.v_active = (int32_t)g->v_active;
   return p;
 }
 
-static hipError_t launch_raw(fc_graph* g, int force_action) {
-  if (g->off64) return launch_phase<int64_t>(make_params<int64_t>(g, force_action), g->blocks, g->stream);
-  return launch_phase<int32_t>(make_params<int32_t>(g, force_action), g->blocks, g->stream);
-}
-
-fc_status launch_phase_once(fc_graph* g, int force_action) {
-  hipError_t e = launch_raw(g, force_action);
-  if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
-  return FC_OK;
+static hipError_t launch_raw(fc_graph* g) {
+  if (g->off64) return launch_phase<int64_t>(make_params<int64_t>(g), g->blocks, g->stream);
+  return launch_phase<int32_t>(make_params<int32_t>(g), g->blocks, g->stream);
 }
 
 static fc_status ensure_graph(fc_graph* g) {
   if (g->gexec) return FC_OK;
   FC_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
-  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = launch_raw(g, -1);
+  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = launch_raw(g);
   hipGraph_t gr = nullptr;
   hipError_t ce = hipStreamEndCapture(g->stream, &gr);
   if (le != hipSuccess) return hip_fail(le, "phase kernel capture");
@@ -126,14 +119,16 @@ static fc_status ensure_graph(fc_graph* g) {
   return FC_OK;
 }
 
-fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const int32_t* t, int32_t n, bool parents,
+fc_status run_visits(fc_graph* g, int mode, const int32_t* s, const int32_t* t, int32_t n, bool parents,
                      double* ms) {
   *ms = 0.0;
   if (n <= 0) return FC_OK;
   fc_status st = ensure_queries(g, n);
   if (st != FC_OK) return st;
-  st = ensure_graph(g);
-  if (st != FC_OK) return st;
+  if (g->use_graphs) {
+    st = ensure_graph(g);
+    if (st != FC_OK) return st;
+  }
   FC_HIP(hipMemcpyAsync(g->d_qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(g->d_qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   VisitCtl init;
@@ -142,7 +137,6 @@ fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const 
   init.qi = 0;
   init.nq = n;
   init.mode = mode;
-  init.policy = policy;
   init.vw_fixed = g->vw_fixed;
   init.record_parents = parents ? 1 : 0;
   init.src = s[0];
@@ -154,19 +148,25 @@ fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const 
   // caller's arrays can go away, and before timing starts.
   FC_HIP(hipStreamSynchronize(g->stream));
 
-  // Launch budget: the running estimate of level phases per query plus the
-  // reset phase. A single BFS needs no extra margin: a short budget costs one
-  // more host poll, and each surplus launch of the full grid about 4 us.
-  const double est = g->est_levels[mode];
-  int64_t expected = (int64_t)std::ceil((double)n * (est + 1.0));
+  // Launch budget: the running estimate of working launches per query (its
+  // reset included). A single BFS needs no extra margin: a short budget costs
+  // one more host poll, and each surplus launch of the full grid about 4 us.
+  // At most kPollLaunches are queued between two polls: a high-diameter
+  // batch needs hundreds of thousands of launches, and queueing them all at
+  // once holds that many dispatches (and a profiler's record of each) in
+  // flight in the runtime. A poll costs one host round trip per
+  // kPollLaunches launches of at least 1.6 us each.
+  const double est = g->est_launches[mode];
+  int64_t expected = (int64_t)std::ceil((double)n * est);
   int64_t launched = 0;
   FC_HIP(hipEventRecord(g->ev0, g->stream));
   for (int round = 0;; ++round) {
     int64_t want = expected - launched;
     if (want < 1) want = 1;
-    if (want < kGraphLaunches) {
+    if (want > kPollLaunches) want = kPollLaunches;
+    if (want < kGraphLaunches || !g->use_graphs) {
       for (int64_t i = 0; i < want; ++i) {
-        hipError_t e = launch_raw(g, -1);
+        hipError_t e = launch_raw(g);
         if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
       }
       launched += want;
@@ -184,18 +184,20 @@ fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const 
     // never reached by a working visit.
     if (launched > (int64_t)n * (g->V + 4) + 64)
       return fail(FC_ERR_CAPACITY, "phase machine did not finish after %lld launches", (long long)launched);
-    // Not finished: budget the remaining queries with a wider margin.
+    // Not finished. While the budget lasts, keep to it; past it, budget the
+    // remaining queries with a wider margin.
+    if (launched < expected) continue;
     const int64_t remaining = std::max<int64_t>(n - qi, 1);
-    expected = launched + (int64_t)std::ceil((double)remaining * (est + 2.0) * (1.0 + round)) + 1;
+    expected = launched + (int64_t)std::ceil((double)remaining * (est + 1.0) * (1.0 + round)) + 1;
   }
   float fms = 0.f;
   FC_HIP(hipEventElapsedTime(&fms, g->ev0, g->ev1));
   *ms = fms;
   FC_HIP(hipMemcpyAsync(g->h_out, g->d_out, sizeof(VisitOut) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
   FC_HIP(hipStreamSynchronize(g->stream));
-  double lv = 0;
-  for (int32_t i = 0; i < n; ++i) lv += g->h_out[i].levels;
-  g->est_levels[mode] = 0.5 * g->est_levels[mode] + 0.5 * (lv / n);
+  double wl = 0;
+  for (int32_t i = 0; i < n; ++i) wl += g->h_out[i].launches;
+  g->est_launches[mode] = 0.5 * g->est_launches[mode] + 0.5 * (wl / n);
   return FC_OK;
 }
 
@@ -221,6 +223,12 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   if (ht && std::atoi(ht) > 0) g->hub_threshold = std::atoi(ht);
   const char* vwf = std::getenv("FASTCON_VW");
   if (vwf && std::atoi(vwf) > 0) g->vw_fixed = std::atoi(vwf);
+  // Direct launches instead of replayed hipGraphs, for profiler runs: under
+  // rocprofv3's kernel trace a hipGraphLaunch of the visit's graph ended in
+  // a host segmentation fault inside the runtime, while direct launches of
+  // the same kernel trace normally.
+  const char* pg = std::getenv("FASTCON_PLAIN_GRAPHS");
+  if (pg && std::strcmp(pg, "0") == 0) g->use_graphs = false;
   int64_t hubs = 0;
   fc_status st = sizeof(OffT) == 8
                      ? fc::validate_csr64((const int64_t*)off, col, V, E, g->hub_threshold, &hubs)
@@ -228,6 +236,14 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   if (st != FC_OK) {
     delete g;
     return st;
+  }
+  // Hub-queue counters pack (entries << 40) | edges into 64 bits: at most
+  // 2^24 - 1 hub entries per frontier.
+  if (hubs >= (int64_t)(1ll << (64 - fcd::kHubEdgeBits))) {
+    const int32_t thr = g->hub_threshold;
+    delete g;
+    return fail(FC_ERR_CAPACITY, "fc_graph_create: %lld vertices of degree >= %d exceed the hub queue's %lld entries",
+                (long long)hubs, thr, (long long)((1ll << (64 - fcd::kHubEdgeBits)) - 1));
   }
   auto bail = [&](fc_status s) {
     fc_graph_destroy(g);
@@ -369,7 +385,6 @@ extern "C" fc_status fc_graph_destroy(fc_graph* g) {
   (void)hipFree(g->d_qt);
   (void)hipFree(g->d_out);
   (void)hipFree(g->d_mlp);
-  (void)hipFree(g->d_snap);
   (void)hipFree(g->d_ws);
   (void)hipFree(g->d_col);
   (void)hipFree(g->d_off);
@@ -394,6 +409,7 @@ static void add_stats(const fcd::VisitOut& o, fc_stats* s) {
   s->topdown_levels += o.topdown;
   s->bottomup_levels += o.bottomup;
   s->max_frontier = std::max(s->max_frontier, o.max_frontier);
+  s->launches += o.launches;
 }
 
 static fc_status check_out(const fcd::VisitOut& o) {
@@ -430,7 +446,7 @@ extern "C" fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32
   const int32_t dsrc = g->relabeled ? g->h_perm[(size_t)src] : src;
   double ms = 0;
   g->levels_valid = g->parents_valid = false;
-  fc_status st = fcd::run_visits(g, fcd::kModeBfs, 0, &dsrc, &none, 1, par, &ms);
+  fc_status st = fcd::run_visits(g, fcd::kModeBfs, &dsrc, &none, 1, par, &ms);
   if (st != FC_OK) return st;
   const fcd::VisitOut o = g->h_out[0];
   st = check_out(o);
@@ -483,8 +499,8 @@ extern "C" fc_status fc_graph_parents(fc_graph* g, int32_t* parents_out) {
   return FC_OK;
 }
 
-static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int mode, int policy,
-                           int32_t* connected, int32_t* dist, int32_t* meet, fc_stats* stats) {
+static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t* connected,
+                           int32_t* dist, int32_t* meet, fc_stats* stats) {
   fc::clear_error();
   if (!g || (n > 0 && (!s || !t))) return fail(FC_ERR_INVALID, "s–t query: null argument");
   if (n < 0) return fail(FC_ERR_INVALID, "s–t query: negative count");
@@ -504,7 +520,7 @@ static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
       idx.push_back(i);
     }
   double ms = 0;
-  fc_status st = fcd::run_visits(g, mode, policy, vs.data(), vt.data(), (int32_t)vs.size(), false, &ms);
+  fc_status st = fcd::run_visits(g, fcd::kModePlainSt, vs.data(), vt.data(), (int32_t)vs.size(), false, &ms);
   if (st != FC_OK) return st;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -530,12 +546,12 @@ static fc_status stcon_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
 
 extern "C" fc_status fc_stcon(fc_graph* g, int32_t s, int32_t t, int32_t* connected, int32_t* dist, int32_t* meet,
                               fc_stats* stats) {
-  return stcon_run(g, &s, &t, 1, fcd::kModePlainSt, 0, connected, dist, meet, stats);
+  return stcon_run(g, &s, &t, 1, connected, dist, meet, stats);
 }
 
 extern "C" fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t* connected,
                                    int32_t* dist, fc_stats* stats) {
-  return stcon_run(g, s, t, n, fcd::kModePlainSt, 0, connected, dist, nullptr, stats);
+  return stcon_run(g, s, t, n, connected, dist, nullptr, stats);
 }
 
 // Shortest s–t path from the parents a plain visit recorded (one wave).
@@ -627,7 +643,7 @@ extern "C" fc_status fc_stcon_path(fc_graph* g, int32_t s, int32_t t, int32_t* p
   const int32_t ds = g->relabeled ? g->h_perm[(size_t)s] : s;
   const int32_t dt = g->relabeled ? g->h_perm[(size_t)t] : t;
   double ms = 0;
-  fc_status st = fcd::run_visits(g, fcd::kModePlainSt, 0, &ds, &dt, 1, true, &ms);
+  fc_status st = fcd::run_visits(g, fcd::kModePlainSt, &ds, &dt, 1, true, &ms);
   if (st != FC_OK) return st;
   const fcd::VisitOut o = g->h_out[0];
   st = check_out(o);
@@ -710,11 +726,10 @@ extern "C" fc_status fc_stcon_sched_many(fc_graph* g, const int32_t* s, const in
   if (policy == FC_POLICY_PLAIN) return fc_stcon_many(g, s, t, n, connected, dist, stats);
   if (policy != FC_POLICY_HEURISTIC && policy != FC_POLICY_LEARNED)
     return fail(FC_ERR_INVALID, "unknown policy %d", policy);
-  // One-sided scheduled steps for single queries stay switched off: an
-  // earlier persistent-kernel version of this path waited out its
-  // grid-barrier bound on the GPU and that cause was not identified.
-  // Heuristic and learned scheduling run in the batched engine.
-  (void)stcon_run;
+  // One-sided scheduled steps for single queries are not built: an earlier
+  // persistent-kernel version of this path waited out its grid-barrier
+  // bound on the GPU, that cause was not identified, and the code was
+  // removed. Heuristic and learned scheduling run in the batched engine.
   return fail(FC_ERR_UNSUPPORTED,
               "scheduled single-query visits are switched off; use fc_stcon_batch for heuristic or "
               "learned scheduling");

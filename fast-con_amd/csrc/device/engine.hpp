@@ -30,10 +30,9 @@ struct fc_graph {
   uint32_t hcap = 1;
   uint32_t qcap = 1;
 
-  // Visit workspace: one allocation, so a step-by-step trace can snapshot it.
+  // Visit workspace: one allocation.
   char* d_ws = nullptr;
   size_t ws_bytes = 0;
-  char* d_snap = nullptr;       // lazily allocated copy used by the tracer
   int32_t* d_state = nullptr;
   int32_t* d_parent = nullptr;
   uint32_t* d_visited = nullptr;
@@ -54,8 +53,9 @@ struct fc_graph {
   // Captured hipGraph of kGraphLaunches identical phase launches.
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  double est_levels[3] = {12.0, 4.0, 4.0};  // running estimate per mode
+  double est_launches[2] = {9.0, 5.0};  // working launches per query (running estimate, per mode)
   int32_t vw_fixed = 0;
+  bool use_graphs = true;           // replay captured launches (FASTCON_PLAIN_GRAPHS=0: direct launches)
 
   // Degree relabelling (absent with FC_GRAPH_KEEP_IDS): the device graph
   // uses new ids; perm maps a caller id to its device id, iperm back.
@@ -81,16 +81,15 @@ struct fc_graph {
 namespace fcd {
 
 constexpr int kGraphLaunches = 16;
+constexpr int64_t kPollLaunches = 4096;  // launches queued between two host polls of the phase word
 
 fc_status hip_fail(hipError_t e, const char* what);
 fc_status ensure_queries(fc_graph* g, int32_t n);
 
 // Run n visits through the phase machine (s[i], t[i]; t = -1 for BFS) and
 // leave their results in g->h_out[0..n). *ms receives the device time.
-fc_status run_visits(fc_graph* g, int mode, int policy, const int32_t* s, const int32_t* t, int32_t n,
-                     bool parents, double* ms);
-// One phase launch with an optional forced scheduler action (tracing).
-fc_status launch_phase_once(fc_graph* g, int force_action);
+fc_status run_visits(fc_graph* g, int mode, const int32_t* s, const int32_t* t, int32_t n, bool parents,
+                     double* ms);
 
 template <typename OffT>
 hipError_t launch_phase(const VisitParams<OffT>& p, int blocks, hipStream_t s);

@@ -493,20 +493,6 @@ __device__ __forceinline__ int degree_width(uint32_t n, uint64_t mf_normal) {
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
 
-// Scheduler action a = side * 3 + k: k=0 top-down width 8, k=1 top-down
-// width 64, k=2 bottom-up.
-__device__ __forceinline__ void apply_action(Decision& d, int a) {
-  const int side = a / 3, k = a % 3;
-  const int sg = k == 2 ? kBottomUp : kTopDown;
-  const int w = k == 1 ? 64 : 8;
-  d.ex[0] = side == 0;
-  d.ex[1] = side == 1;
-  d.strat[0] = side == 0 ? sg : kTopDown;
-  d.strat[1] = side == 1 ? sg : kTopDown;
-  d.vw[0] = side == 0 ? w : 8;
-  d.vw[1] = side == 1 ? w : 8;
-}
-
 // Decision for the next level launch (one thread of the finalising block).
 __device__ void decide(VisitCtl* ctl, int64_t V, int64_t E) {
   Decision d;
@@ -536,17 +522,10 @@ __device__ void decide(VisitCtl* ctl, int64_t V, int64_t E) {
     }
     d.strat[0] = ctl->bu_mode ? kBottomUp : kTopDown;
     d.vw[0] = width(0);
-  } else if (mode == kModePlainSt) {
+  } else {
     d.ex[0] = d.ex[1] = 1;
     d.vw[0] = width(0);
     d.vw[1] = width(1);
-  } else {
-    // Scheduled single-query visits (switched off on the host side; the
-    // learned scheduler drives the batched engine): the side with the
-    // smaller frontier, bottom-up when it would touch most unseen edges.
-    const int side = cmf[0] <= cmf[1] ? 0 : 1;
-    const bool bu = (double)cmf[side] > mu / kAlpha;
-    apply_action(d, side * 3 + (bu ? 2 : (width(side) >= 16 ? 1 : 0)));
   }
   ctl->dec = d;
 }
@@ -575,6 +554,7 @@ __device__ void finalize_reset(const VisitParams<OffT>& p) {
   }
   ctl->lvl[0] = ctl->lvl[1] = 0;
   ctl->steps = ctl->td = ctl->bu = ctl->maxf = 0;
+  ctl->launches = 1;
   ctl->bu_mode = 0;
   ctl->best = ~0ull;
   ctl->err = 0u;
@@ -591,6 +571,7 @@ __device__ void finalize_level(const VisitParams<OffT>& p, const Decision& d) {
   VisitCtl* ctl = p.ctl;
   const bool st = ctl->mode != kModeBfs;
   ctl->steps++;
+  ctl->launches++;
   bool empty = false;
   for (int c = 0; c < 2; ++c) {
     if (!d.ex[c]) continue;
@@ -633,7 +614,7 @@ __device__ void finalize_level(const VisitParams<OffT>& p, const Decision& d) {
     o.lvl_end[0] = ctl->lvl[0];
     o.lvl_end[1] = ctl->lvl[1];
     o.err = (int32_t)err;
-    o.pad = 0;
+    o.launches = ctl->launches;
     p.out[ctl->qi] = o;
     ctl->qi++;
     if (ctl->qi < ctl->nq) {
@@ -703,7 +684,6 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
       }
     }
   } else {
-    if (p.force_action >= 0) apply_action(d, p.force_action);
     const int lvl[2] = {ctl->lvl[0], ctl->lvl[1]};
 #pragma unroll
     for (int c = 0; c < 2; ++c) {

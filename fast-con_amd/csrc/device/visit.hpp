@@ -17,7 +17,7 @@ constexpr int32_t kLevelMask = kColorBit - 1;
 constexpr uint64_t kHubEdgeBits = 40;         // hub counter: (entries << 40) | edges
 constexpr uint64_t kHubEdgeMask = (1ull << kHubEdgeBits) - 1;
 
-enum VisitMode : int32_t { kModeBfs = 0, kModePlainSt = 1, kModeSchedSt = 2 };
+enum VisitMode : int32_t { kModeBfs = 0, kModePlainSt = 1 };
 enum Strategy : int32_t { kTopDown = 0, kBottomUp = 1 };
 enum Phase : int32_t { kPhaseIdle = 0, kPhaseReset = 1, kPhaseLevel = 2 };
 
@@ -55,7 +55,7 @@ struct VisitOut {
   int64_t expanded_edges;
   int32_t lvl_end[2];
   int32_t err;
-  int32_t pad;
+  int32_t launches;          // working launches of this query (its reset included)
 };
 
 // Control block of the phase machine. Written by the finalising block of one
@@ -65,10 +65,10 @@ struct VisitCtl {
   int32_t qi;               // current query
   int32_t nq;               // number of queries
   int32_t mode;
-  int32_t policy;           // scheduled mode: 1 heuristic, 2 learned
+  int32_t launches;         // working launches of the current query
   int32_t vw_fixed;         // 0: width from frontier degree
   int32_t record_parents;
-  int32_t max_steps;        // 0: unlimited (a traced query stops early)
+  int32_t pad1;
   int32_t src, dst;
   int32_t lvl[2];
   int32_t steps, td, bu, maxf;
@@ -82,7 +82,6 @@ struct VisitCtl {
   uint32_t gticket[kTicketGroups * 32];  // group tickets, one 128-byte line each
   uint64_t seen_edges;      // sum of degrees of every vertex labelled so far
   int64_t visited_v, visited_e, expv, expe;
-  float feat[12];           // features of the pending decision (scheduled mode)
 };
 
 constexpr int kFeatures = 12;
@@ -110,31 +109,9 @@ struct VisitParams {
   const int32_t* qs;        // query sources (device), nq entries
   const int32_t* qt;        // query targets
   VisitOut* out;            // results (device), nq entries
-  int32_t force_action;     // -1, or a scheduler action applied to this launch (tracing)
   int32_t lazy_state;       // BFS leaves unreached state/parent unwritten (renumbered graphs:
                             // the un-permutation takes reached-ness from the visited bitmap)
   int32_t v_active;         // vertices [0, v_active) have degree > 0 (the rest are isolated)
-  int32_t pad;
 };
-
-// Scheduler features of a pending step, shared by the kernel and the host tracer.
-__host__ __device__ inline void step_features(float* f, const uint32_t n[2], const uint64_t mf[2],
-                                              const uint64_t hubn[2], int lvl0, int lvl1, int64_t V, int64_t E,
-                                              uint64_t seen_edges) {
-  auto lg = [](double x) { return (float)log2(x + 1.0); };
-  const double mu = (double)E - (double)seen_edges;
-  f[0] = lg((double)n[0]) / 32.f;
-  f[1] = lg((double)mf[0]) / 40.f;
-  f[2] = lg((double)n[1]) / 32.f;
-  f[3] = lg((double)mf[1]) / 40.f;
-  f[4] = lg(mu > 0 ? mu : 0) / 40.f;
-  f[5] = lg((double)V) / 32.f;
-  f[6] = lg((double)E) / 40.f;
-  f[7] = (float)(lvl0 < 64 ? lvl0 : 64) / 64.f;
-  f[8] = (float)(lvl1 < 64 ? lvl1 : 64) / 64.f;
-  f[9] = lg(n[0] ? (double)mf[0] / n[0] : 0.0) / 20.f;
-  f[10] = lg(n[1] ? (double)mf[1] / n[1] : 0.0) / 20.f;
-  f[11] = (float)(hubn[0] + hubn[1]) / (float)(n[0] + n[1] + 1);
-}
 
 }  // namespace fcd

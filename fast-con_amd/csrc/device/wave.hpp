@@ -1,12 +1,12 @@
-// Wave64 primitives for gfx950 and the bounded grid barrier used by the
-// persistent level-loop kernels.
+// Wave64 primitives for gfx950.
 //
 // These replace the reference's 32-lane PTX scans and lane masks
 // (BFS-4K/src/device/Util/prefixSumAsm.cu:5-139, BFS-4K/src/device/Util/ptx.cu:6-85),
-// its warp-aggregated frontier reservation (BFS-4K/src/device/Util/GlobalWrite.cu:13-46)
-// and its software grid barrier (BFS-4K/src/device/Util/GlobalSync.cu:5-36,
-// which spun on cub::ThreadLoad<LOAD_CG>). On CDNA a wave has 64 lanes, so
-// ballots are 64-bit and ranks come from v_mbcnt_lo/hi.
+// and its warp-aggregated frontier reservation (BFS-4K/src/device/Util/GlobalWrite.cu:13-46).
+// On CDNA a wave has 64 lanes, so ballots are 64-bit and ranks come from
+// v_mbcnt_lo/hi. The reference's software grid barrier
+// (BFS-4K/src/device/Util/GlobalSync.cu:5-36) has no counterpart: levels are
+// separated by kernel launches (see visit.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -72,61 +72,6 @@ __device__ __forceinline__ T ld_agent(const T* p) {
 template <typename T>
 __device__ __forceinline__ void st_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier state in device memory; zeroed before each launch.
-struct GridBar {
-  uint32_t count;
-  uint32_t gen;
-  uint32_t abort;
-  uint32_t pad;
-};
-
-// Sense-counting grid barrier for a cooperative launch (all blocks resident).
-// Each wait is bounded by `limit` ticks of the 100 MHz real-time counter; on
-// expiry, or when another block already gave up, it sets `abort` and returns
-// false so every caller leaves its level loop and the grid drains.
-// Release/acquire fences at agent scope make the plain stores of one phase
-// visible to the plain loads of the next across XCDs. Loop-exit decisions
-// taken after the barrier read only values completed before it, so every
-// block takes the same decision.
-__device__ __forceinline__ bool grid_sync(GridBar* b, uint32_t nblocks, uint64_t limit) {
-  __shared__ int s_ok;
-  // Every wave first waits for its own outstanding memory operations,
-  // including non-returning atomics (atomicMin/atomicAdd whose result is
-  // unused), so that the block's arrival below publishes all of them.
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    const uint32_t g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t arrived =
-        __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nblocks - 1) {
-      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const uint64_t t0 = wall_clock64();
-      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          ok = 0;
-          break;
-        }
-        if (wall_clock64() - t0 > limit) {
-          __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
-    s_ok = ok;
-  }
-  __syncthreads();
-  return s_ok != 0;
 }
 
 }  // namespace fcd

@@ -12,9 +12,11 @@ Reference interfaces mirrored (paths relative to the reference checkout):
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import json
 import os
+import weakref
 from dataclasses import dataclass
 from typing import Optional
 
@@ -59,10 +61,10 @@ class Stats(C.Structure):
                 ("visited_vertices", C.c_int64), ("visited_edges", C.c_int64),
                 ("expanded_vertices", C.c_int64), ("expanded_edges", C.c_int64),
                 ("topdown_levels", C.c_int32), ("bottomup_levels", C.c_int32),
-                ("max_frontier", C.c_int32), ("reserved", C.c_int32)]
+                ("max_frontier", C.c_int32), ("launches", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class BatchOpts(C.Structure):
@@ -283,6 +285,20 @@ def read_edge_list(path: str):
     return arr.reshape(-1, 2), nv.value
 
 
+# Device graphs still open. They are destroyed when the interpreter exits,
+# from an atexit hook: Python runs those before the C runtime's exit
+# handlers, so the HIP runtime is still up when their streams, graphs and
+# buffers are released. A graph left to the garbage collector at shutdown
+# could otherwise be released after the runtime has been torn down.
+_live_graphs = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_graphs():
+    for g in list(_live_graphs):
+        g.close()
+
+
 @dataclass
 class StResult:
     connected: np.ndarray
@@ -309,11 +325,13 @@ class Graph:
             _check(_lib.fc_graph_create_csr64(_p64(hg.offsets), _p32(hg.columns), hg.V, hg.E, flags,
                                               C.byref(self._h)))
             self.offsets64 = True
+        _live_graphs.add(self)
 
     def close(self):
         if self._h:
             _lib.fc_graph_destroy(self._h)
             self._h = C.c_void_p()
+        _live_graphs.discard(self)
 
     def __del__(self):
         try:

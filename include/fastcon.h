@@ -191,7 +191,7 @@ typedef struct fc_stats {
   int32_t topdown_levels;
   int32_t bottomup_levels;
   int32_t max_frontier;
-  int32_t reserved;
+  int32_t launches;           /* working kernel launches of the visit (each query's reset included) */
 } fc_stats;
 
 /* Single-source BFS. Replaces Reset + cudaBFS4K + readback
@@ -218,6 +218,7 @@ fc_status fc_stcon(fc_graph* g, int32_t s, int32_t t, int32_t* connected, int32_
  * st-conn/make_stcon.c:418-452). */
 fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
                         int32_t* connected, int32_t* dist, fc_stats* stats);
+
 
 /* One plain query with a shortest path, in caller ids: path_out[0] = s,
  * path_out[dist] = t and *len_out = dist + 1 (0 when not connected, 1 when
@@ -251,10 +252,10 @@ fc_status fc_stcon_seeds(fc_graph* g, int32_t s, int32_t t, int32_t helpers, uin
  * otherwise. */
 fc_status fc_validate_stcon(fc_graph* g, int32_t s, int32_t t, int32_t mn, int32_t depth);
 
-/* Plain, heuristic or learned scheduling of one-query-per-launch visits
- * (policy: fc_policy). FC_POLICY_PLAIN is fc_stcon_many. The heuristic and
- * learned policies expand one side per step (top-down or bottom-up) and give
- * the same verdicts and distances. */
+/* Scheduled one-query-per-launch visits. Only FC_POLICY_PLAIN is built (it
+ * is fc_stcon_many); FC_POLICY_HEURISTIC and FC_POLICY_LEARNED return
+ * FC_ERR_UNSUPPORTED. Heuristic and learned scheduling are available in the
+ * batched engine, fc_stcon_batch. */
 fc_status fc_stcon_sched_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy,
                               int32_t* connected, int32_t* dist, fc_stats* stats);
 

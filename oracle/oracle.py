@@ -160,22 +160,48 @@ def batch_wave(off, col, s, t):
     return dist, a.value, e.value, k.value
 
 
-def time_stcon(off, col, s, t):
+def time_stcon(off, col, s, t, lib=None):
     """Single-thread host s–t (BFS from s, level of t). Returns (seconds, dist)."""
+    lib = lib or _lib
     off, col = _csr(off, col)
     s = np.ascontiguousarray(s, np.int32)
     t = np.ascontiguousarray(t, np.int32)
     dist = np.empty(len(s), np.int32)
-    sec = _lib.orc_time_stcon(len(off) - 1, int(off[-1]), _p64(off), _p32(col), len(s), _p32(s), _p32(t), _p32(dist))
+    sec = lib.orc_time_stcon(len(off) - 1, int(off[-1]), _p64(off), _p32(col), len(s), _p32(s), _p32(t), _p32(dist))
     return sec, dist
 
 
-def time_bfs(off, col, roots):
+def time_bfs(off, col, roots, lib=None):
+    """Single-thread host BFS from each root. Returns (seconds, sum of the
+    visitedEdges counts)."""
+    lib = lib or _lib
     off, col = _csr(off, col)
     roots = np.ascontiguousarray(roots, np.int32)
     tot = C.c_int64()
-    sec = _lib.orc_time_bfs(len(off) - 1, int(off[-1]), _p64(off), _p32(col), len(roots), _p32(roots), C.byref(tot))
+    sec = lib.orc_time_bfs(len(off) - 1, int(off[-1]), _p64(off), _p32(col), len(roots), _p32(roots), C.byref(tot))
     return sec, tot.value
+
+
+def native_build(workdir):
+    """The CPU baseline's build: oracle.c compiled with -O3 -march=native for
+    the CPU this runs on, into workdir. Returns (lib, flags) for the lib=
+    argument of time_stcon / time_bfs, or (None, reason) when gcc fails (the
+    prebuilt -O3 liboracle.so is then the one to use)."""
+    import subprocess
+
+    flags = ["-O3", "-march=native", "-std=gnu11", "-fPIC", "-shared"]
+    out = os.path.join(workdir, "liboracle_native.so")
+    try:
+        subprocess.run(["gcc", *flags, "-o", out, os.path.join(_HERE, "oracle.c"), "-lm"], check=True,
+                       capture_output=True, timeout=120)
+    except (OSError, subprocess.SubprocessError) as e:
+        return None, f"gcc failed: {e}"
+    lib = C.CDLL(out)
+    lib.orc_time_stcon.restype = C.c_double
+    lib.orc_time_stcon.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, C.c_int, _i32p, _i32p, _i32p]
+    lib.orc_time_bfs.restype = C.c_double
+    lib.orc_time_bfs.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, C.c_int, _i32p, _i64p]
+    return lib, "gcc " + " ".join(flags[:2])
 
 
 # ---------------------------------------------------------------------------

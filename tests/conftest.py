@@ -53,3 +53,37 @@ def small_graphs(fc):
     out.append(("rmat10", fc.gen_rmat(10, 16, seed=5)))
     out.append(("rmat12-skew", fc.gen_rmat(12, 8, a=0.65, b=0.15, c=0.15, seed=6)))
     return out
+
+
+def oracle_pair_dists(orc, hg, s, t, workers=16, chunk=8):
+    """Host-BFS s–t distances (BFS from s[i], level of t[i]) for many pairs,
+    in a thread pool: the oracle's C calls release the GIL."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    s = np.ascontiguousarray(s, np.int32)
+    t = np.ascontiguousarray(t, np.int32)
+    out = np.empty(len(s), np.int32)
+    workers = max(1, min(workers, os.cpu_count() or 1))
+
+    def run(k):
+        _, d = orc.time_stcon(hg.offsets, hg.columns, s[k:k + chunk], t[k:k + chunk])
+        out[k:k + chunk] = d
+
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(run, range(0, len(s), chunk)))
+    return out
+
+
+def oracle_bfs_many(orc, hg, roots, fn, workers=16):
+    """fn(root, dist, info) for the host BFS of every root, in a thread pool;
+    returns the list of fn's results in root order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    workers = max(1, min(workers, os.cpu_count() or 1))
+
+    def run(r):
+        d, _, info = orc.bfs(hg.offsets, hg.columns, int(r))
+        return fn(int(r), d, info)
+
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(run, roots))

@@ -273,3 +273,38 @@ def test_python_constants_match_the_header(fc):
         assert int(defs[name]) == val, (name, defs[name], val)
     F, H, A = fc.SCHED_FEATURES, fc.SCHED_HIDDEN, fc.SCHED_ACTIONS
     assert fc.SCHED_PARAMS == F * H + H + H * H + H + H * A + A
+
+
+def test_generators_match_golden_fixtures(fc):
+    """The seeded graphs of the tests and the benchmark (R-MAT s16 and s22,
+    the 4096^2 grid with 2% deletions, ER 2^20 with mean degree 8) equal the
+    committed fixtures bit for bit (tests/golden/make_generator_fixtures.py),
+    and their counts obey what the construction fixes: a grid keeps each of
+    its 2n(n-1) undirected edges with probability 1-p (binomial, checked
+    within 6 standard deviations) and an ER graph has at most n*d/2
+    undirected edges."""
+    import json
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_generator_fixtures as mk
+
+    fix = json.load(open(os.path.join(ROOT, "tests", "golden", "generators.json")))
+    assert [c["name"] for c in fix] == [c["name"] for c in mk.CONFIGS]
+    for c in fix:
+        hg = mk.build(fc, c)
+        got = mk.describe(hg)
+        for k, v in got.items():
+            assert v == c[k], f"{c['name']}: {k} is {v}, fixture {c[k]}"
+        assert sum(got["log2_degree_hist"]) == got["V"] - got["isolated"]
+        if c["kind"] == "grid":
+            n_und = 2 * c["rows"] * (c["cols"] - 1)
+            p = c["p_delete"]
+            mu, sd = n_und * (1 - p), (n_und * p * (1 - p)) ** 0.5
+            assert abs(got["E"] / 2 - mu) < 6 * sd
+        if c["kind"] == "er":
+            assert got["E"] <= c["n"] * c["mean_degree"]
+    rmat22 = [c for c in fix if c["name"] == "rmat22"][0]
+    assert rmat22["E"] == 128302336  # the benchmark graph
+    g = fc.gen_grid(64, 64, 0.0, 3)
+    assert g.E == 4 * 64 * 63  # no deletions: every undirected edge in both directions

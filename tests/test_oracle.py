@@ -187,3 +187,77 @@ def test_known_answer_s16(orc):
         assert c == (mn > 0) and dp == depth
         maxd = max(maxd, dp)
     assert maxd == fx["max_level"]
+
+
+def test_count_visited_edges_hand_case(orc):
+    """SC/adj_func.c:334-357 counts, for each reached vertex v, its
+    adjacency entries u <= v. Hand-worked: undirected edges (0,1), (1,2), two
+    self-loops (2,2) and (0,0) kept by Graph::ToCSR, and a second component
+    {3,4,5} the BFS from 0 never reaches. Reached lists: 0 -> [1,0,0],
+    1 -> [0,2], 2 -> [1,2,2], so 2 + 1 + 3 = 6 (each non-loop edge once,
+    each self-loop's two entries twice). Writing < for <= gives 2, and
+    counting unreached vertices too gives 8."""
+    coo = np.array([[0, 1], [1, 2], [2, 2], [0, 0], [3, 4], [4, 5]], np.int32)
+    off, col = orc.to_csr(6, coo, True)
+    lv, _, _ = orc.bfs(off, col, 0)
+    assert list(lv) == [0, 1, 2, -1, -1, -1]
+    assert orc.count_visited_edges(off, col, lv) == 6
+    lv3, _, _ = orc.bfs(off, col, 3)
+    assert orc.count_visited_edges(off, col, lv3) == 2  # edges (3,4) and (4,5) once each
+
+
+def test_batch_wave_closed_form_counters(fc, orc):
+    """The bit-parallel wave on graphs whose frontiers are known in closed
+    form. Path 0..n-1, one query (0, n-1): iteration k expands vertex k-1 on
+    the s side and n-k on the t side, so with m = ceil((n-1)/2) iterations
+    A = 2m frontier (vertex, side) pairs and E = 4m - 2 edges (both ends
+    have degree 1, inner vertices 2), and the distance is n-1. Star with
+    centre 0 and 20 leaves plus an isolated vertex 21: leaf-to-leaf queries
+    meet at the centre in one iteration (distance 2, even), centre-to-leaf
+    queries in one (distance 1, odd); a query to the isolated vertex is not
+    connected after one iteration, the t side having gained nothing."""
+    for n in range(2, 13):
+        e = np.array([[i, i + 1] for i in range(n - 1)])
+        g = fc.csr_from_edges(e, n)
+        dist, A, E, k = orc.batch_wave(g.offsets, g.columns, np.array([0], np.int32), np.array([n - 1], np.int32))
+        m = n // 2
+        assert (dist[0], A, E, k) == (n - 1, 2 * m, 4 * m - 2, m), n
+    e = np.array([[0, i] for i in range(1, 21)])
+    g = fc.csr_from_edges(e, 22)
+    s = np.array([1, 2, 3, 0, 4], np.int32)
+    t = np.array([5, 6, 7, 9, 21], np.int32)
+    dist, A, E, k = orc.batch_wave(g.offsets, g.columns, s, t)
+    assert list(dist) == [2, 2, 2, 1, -1] and k == 1
+    # frontier (vertex, side) pairs: s side {1,2,3,0,4}, t side {5,6,7,9,21}
+    assert A == 10
+    # degrees: leaves 1, centre 20, isolated 0
+    assert E == 4 * 1 + 20 + 4 * 1 + 0
+
+
+def test_validate_bfs_tree_rules_3_and_4(fc, orc):
+    """Rule 4: an unreached vertex must have no parent. Rule 3: an edge may
+    not join a reached and an unreached vertex, nor two levels more than one
+    apart."""
+    e = np.array([[0, 1], [1, 2], [2, 3], [5, 6]])
+    g = fc.csr_from_edges(e, 7)
+    lv, _, _ = orc.bfs(g.offsets, g.columns, 0)
+    par = np.array([0, 0, 1, 2, -1, -1, -1], np.int32)
+    assert orc.validate_bfs_tree(g.offsets, g.columns, 0, lv, par) == 0
+    bad = par.copy()
+    bad[5] = 6  # an unreached vertex with a parent
+    assert orc.validate_bfs_tree(g.offsets, g.columns, 0, lv, bad) == 4
+    lv2, par2 = lv.copy(), par.copy()
+    lv2[3], par2[3] = -1, -1  # 3 dropped from the tree while its neighbour 2 is in it
+    assert orc.validate_bfs_tree(g.offsets, g.columns, 0, lv2, par2) == 3
+    e4 = np.array([[0, 1], [1, 2], [2, 3], [0, 3]])
+    g4 = fc.csr_from_edges(e4, 4)
+    lv4, _, _ = orc.bfs(g4.offsets, g4.columns, 0)
+    assert list(lv4) == [0, 1, 2, 1]
+    par4 = np.array([0, 0, 1, 0], np.int32)
+    assert orc.validate_bfs_tree(g4.offsets, g4.columns, 0, lv4, par4) == 0
+    # vertex 3 claimed at level 3 through parent 2: the tree edge is fine but
+    # the edge (0,3) now joins levels 0 and 3
+    lv4b = lv4.copy()
+    lv4b[3] = 3
+    par4b = np.array([0, 0, 1, 2], np.int32)
+    assert orc.validate_bfs_tree(g4.offsets, g4.columns, 0, lv4b, par4b) == 3
