@@ -118,6 +118,8 @@ _sig("fc_graph_levels", C.c_int, C.c_void_p, _i32p)
 _sig("fc_graph_parents", C.c_int, C.c_void_p, _i32p)
 _sig("fc_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.POINTER(Stats))
 _sig("fc_stcon_many", C.c_int, C.c_void_p, _i32p, _i32p, C.c_int32, _i32p, _i32p, C.POINTER(Stats))
+_sig("fc_stcon_many_static", C.c_int, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, _i32p, _i32p, C.c_int32, _i32p,
+     _i32p, C.POINTER(Stats))
 _sig("fc_stcon_path", C.c_int, C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int32, _i32p, C.POINTER(Stats))
 _sig("fc_stcon_seeds", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _i32p, C.POINTER(Stats))
 _sig("fc_validate_stcon", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32)
@@ -143,7 +145,7 @@ EXPORTED = [
     "fc_graph_parents", "fc_stcon", "fc_stcon_many", "fc_stcon_sched_many", "fc_stcon_batch",
     "fc_sched_trace", "fc_sched_train", "fc_sched_set_params", "fc_sched_get_params", "fc_sched_forward",
     "fc_sched_gradient", "fc_stcon_path", "fc_validate_stcon", "fc_validate_bfs", "fc_gen_rmat_gpu",
-    "fc_csr_from_edges_gpu", "fc_stcon_seeds",
+    "fc_csr_from_edges_gpu", "fc_stcon_seeds", "fc_stcon_many_static",
 ]
 
 
@@ -267,6 +269,20 @@ def csr_from_edges_gpu(edges: np.ndarray, V: int = 0) -> HostGraph:
     hg = _HostGraph()
     _check(_lib.fc_csr_from_edges_gpu(_p64(e), len(e) // 2, V, C.byref(hg)))
     return HostGraph._take(hg)
+
+
+def stcon_many_static(d_off: int, d_col: int, V: int, E: int, s, t) -> StResult:
+    """fc_stcon_many_static: plain s–t queries on a CSR graph in the
+    caller's device memory (d_off, d_col: device addresses of V+1 and E
+    int32 values), with no device allocation by the call."""
+    s = np.ascontiguousarray(s, np.int32)
+    t = np.ascontiguousarray(t, np.int32)
+    conn = np.empty(len(s), np.int32)
+    dist = np.empty(len(s), np.int32)
+    st = Stats()
+    _check(_lib.fc_stcon_many_static(C.c_void_p(d_off), C.c_void_p(d_col), V, E, _p32(s), _p32(t), len(s),
+                                     _p32(conn), _p32(dist), C.byref(st)))
+    return StResult(conn, dist, st.as_dict())
 
 
 def read_graph(path: str, direction: int = DIR_FILE) -> HostGraph:

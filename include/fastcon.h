@@ -219,6 +219,23 @@ fc_status fc_stcon(fc_graph* g, int32_t s, int32_t t, int32_t* connected, int32_
 fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n,
                         int32_t* connected, int32_t* dist, fc_stats* stats);
 
+/* fc_stcon_many without any device allocation. No counterpart in the
+ * reference (whose ST-CON wrappers cudaMalloc and cudaFree scratch on every
+ * call); it is for callers that cannot allocate device memory, such as
+ * allocation-free serving loops and harnesses that own all device memory.
+ * d_off (V+1 int32) and d_col (E int32) are a CSR graph in the caller's
+ * device memory, used in the caller's numbering and adjacency order; s, t,
+ * connected, dist and stats are host memory as in fc_stcon_many, with the
+ * same verdicts and distances. The per-query workspace is one fixed device
+ * array of about 23 MB that every GPU the library runs on holds for the life
+ * of the process. Limits: 1 <= V <= 2^20 and E <= 2^25 (FC_ERR_INVALID
+ * beyond them). The call creates no stream, event or graph; it launches on
+ * the null stream and returns when done. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU race and must be serialised.
+ * stats->time_ms is host wall time. */
+fc_status fc_stcon_many_static(const int32_t* d_off, const int32_t* d_col, int32_t V, int64_t E,
+                               const int32_t* s, const int32_t* t, int32_t n, int32_t* connected,
+                               int32_t* dist, fc_stats* stats);
 
 /* One plain query with a shortest path, in caller ids: path_out[0] = s,
  * path_out[dist] = t and *len_out = dist + 1 (0 when not connected, 1 when
