@@ -42,16 +42,10 @@ fc_status hip_fail(hipError_t e, const char* what) {
               "%s: %s", what, hipGetErrorString(e));
 }
 
-static void drop_graph(fc_graph* g) {
-  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  g->gexec = nullptr;
-  g->graph = nullptr;
-}
-
 fc_status ensure_queries(fc_graph* g, int32_t n) {
   if (n <= g->q_cap) return FC_OK;
-  drop_graph(g);  // the captured launches hold the old pointers
+  // the captured launches take only the control block, whose parameters
+  // are rewritten by every run, so the graph stays valid
   (void)hipFree(g->d_qs);
   (void)hipFree(g->d_qt);
   (void)hipFree(g->d_out);
@@ -79,6 +73,7 @@ static VisitParams<OffT> make_params(fc_graph* g) {
   p.E = g->E;
   p.hub_threshold = g->hub_threshold;
   p.state = g->d_state;
+  p.sst = g->d_sst;
   p.parent = g->d_parent;
   p.visited = g->d_visited;
   for (int c = 0; c < 2; ++c) {
@@ -100,16 +95,16 @@ static VisitParams<OffT> make_params(fc_graph* g) {
   return p;
 }
 
-static hipError_t launch_raw(fc_graph* g) {
-  if (g->off64) return launch_phase<int64_t>(make_params<int64_t>(g), g->blocks, g->stream);
-  return launch_phase<int32_t>(make_params<int32_t>(g), g->blocks, g->stream);
+static hipError_t launch_raw(fc_graph* g, int parity) {
+  if (g->off64) return launch_phase<int64_t>(g->d_ctl, parity, g->blocks, g->stream);
+  return launch_phase<int32_t>(g->d_ctl, parity, g->blocks, g->stream);
 }
 
 static fc_status ensure_graph(fc_graph* g) {
   if (g->gexec) return FC_OK;
   FC_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
-  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = launch_raw(g);
+  for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = launch_raw(g, i & 1);  // parities 0, 1, ...
   hipGraph_t gr = nullptr;
   hipError_t ce = hipStreamEndCapture(g->stream, &gr);
   if (le != hipSuccess) return hip_fail(le, "phase kernel capture");
@@ -129,35 +124,38 @@ fc_status run_visits(fc_graph* g, int mode, const int32_t* s, const int32_t* t, 
     st = ensure_graph(g);
     if (st != FC_OK) return st;
   }
+  // s–t queries tag the state words with one epoch each (0 = never
+  // visited); before the counter would wrap, the words are cleared.
+  uint32_t epoch0 = 0;
+  if (mode == kModePlainSt) {
+    if ((uint64_t)g->epoch_next + (uint64_t)n >= 0xFFFFFFF0ull) {
+      FC_HIP(hipMemsetAsync(g->d_sst, 0, sizeof(unsigned long long) * (size_t)g->V, g->stream));
+      g->epoch_next = 1;
+    }
+    epoch0 = g->epoch_next;
+    g->epoch_next += (uint32_t)n;
+  }
   FC_HIP(hipMemcpyAsync(g->d_qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(g->d_qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
-  VisitCtl init;
-  std::memset(&init, 0, sizeof(init));
-  init.phase = kPhaseReset;
-  init.qi = 0;
-  init.nq = n;
-  init.mode = mode;
-  init.vw_fixed = g->vw_fixed;
-  init.record_parents = parents ? 1 : 0;
-  init.src = s[0];
-  init.dst = t[0];
-  init.ticket = 0;
-  init.best = ~0ull;
+  const VisitCtl init = g->off64 ? visit_init(make_params<int64_t>(g), mode, n, s, t, epoch0, g->vw_fixed, parents)
+                                 : visit_init(make_params<int32_t>(g), mode, n, s, t, epoch0, g->vw_fixed, parents);
   FC_HIP(hipMemcpyAsync(g->d_ctl, &init, sizeof(init), hipMemcpyHostToDevice, g->stream));
   // The async copies above read pageable host memory: finish them before the
   // caller's arrays can go away, and before timing starts.
   FC_HIP(hipStreamSynchronize(g->stream));
 
-  // Launch budget: the running estimate of working launches per query (its
-  // reset included). A single BFS needs no extra margin: a short budget costs
-  // one more host poll, and each surplus launch of the full grid about 4 us.
-  // At most kPollLaunches are queued between two polls: a high-diameter
-  // batch needs hundreds of thousands of launches, and queueing them all at
-  // once holds that many dispatches (and a profiler's record of each) in
-  // flight in the runtime. A poll costs one host round trip per
-  // kPollLaunches launches of at least 1.6 us each.
+  // Launch budget: the running estimate of working launches per query, plus
+  // the launch that notices the last query's end. A single BFS needs no
+  // extra margin: a short budget costs one more host poll, and each surplus
+  // launch of the full grid a few us. At most kPollLaunches are queued
+  // between two polls: a high-diameter batch needs hundreds of thousands of
+  // launches, and queueing them all at once holds that many dispatches (and
+  // a profiler's record of each) in flight in the runtime. A poll costs one
+  // host round trip per kPollLaunches launches of at least 1.6 us each.
+  // Launches alternate parity 0, 1, 0, ... (the state a launch reads), and
+  // the captured graphs hold an even number of launches starting at 0.
   const double est = g->est_launches[mode];
-  int64_t expected = (int64_t)std::ceil((double)n * est);
+  int64_t expected = (int64_t)std::ceil((double)n * est) + 1;
   int64_t launched = 0;
   FC_HIP(hipEventRecord(g->ev0, g->stream));
   for (int round = 0;; ++round) {
@@ -166,17 +164,25 @@ fc_status run_visits(fc_graph* g, int mode, const int32_t* s, const int32_t* t, 
     if (want > kPollLaunches) want = kPollLaunches;
     if (want < kGraphLaunches || !g->use_graphs) {
       for (int64_t i = 0; i < want; ++i) {
-        hipError_t e = launch_raw(g);
+        hipError_t e = launch_raw(g, (int)(launched & 1));
         if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
+        ++launched;
       }
-      launched += want;
     } else {
+      if (launched & 1) {
+        hipError_t e = launch_raw(g, 1);
+        if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
+        ++launched;
+        --want;
+      }
       const int64_t ng = (want + kGraphLaunches - 1) / kGraphLaunches;
       for (int64_t i = 0; i < ng; ++i) FC_HIP(hipGraphLaunch(g->gexec, g->stream));
       launched += ng * kGraphLaunches;
     }
     FC_HIP(hipEventRecord(g->ev1, g->stream));
-    FC_HIP(hipMemcpyAsync(g->h_phase, g->d_ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    // the state the next launch would read
+    FC_HIP(hipMemcpyAsync(g->h_phase, &g->d_ctl->st[launched & 1], 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          g->stream));
     FC_HIP(hipStreamSynchronize(g->stream));
     const int32_t phase = g->h_phase[0], qi = g->h_phase[1];
     if (phase == kPhaseIdle) break;
@@ -285,10 +291,10 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
                     g->h_iperm.data(), (int32_t*)noff.data(), ncol.data());
     CK(hipMalloc((void**)&g->d_perm, sizeof(int32_t) * (size_t)V));
     CK(hipMalloc((void**)&g->d_iperm, sizeof(int32_t) * (size_t)V));
-    CK(hipMemcpy(g->d_perm, g->h_perm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice));
-    CK(hipMemcpy(g->d_iperm, g->h_iperm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice));
-    CK(hipMemcpy(g->d_off, noff.data(), sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
-    if (E > 0) CK(hipMemcpy(g->d_col, ncol.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+    CK(hipMemcpyAsync(g->d_perm, g->h_perm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice, g->stream));
+    CK(hipMemcpyAsync(g->d_iperm, g->h_iperm.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice, g->stream));
+    CK(hipMemcpyAsync(g->d_off, noff.data(), sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice, g->stream));
+    if (E > 0) CK(hipMemcpyAsync(g->d_col, ncol.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice, g->stream));
     // decreasing degree: the vertices of degree > 0 are a prefix
     int64_t va = V;
     while (va > 0 && noff[(size_t)va] == noff[(size_t)va - 1]) --va;
@@ -298,18 +304,18 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
     g->v_active = va;
   } else {
     g->v_active = V;
-    CK(hipMemcpy(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice));
+    CK(hipMemcpyAsync(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice, g->stream));
   }
   if (E > 0 && !g->relabeled) {
     if (flags & FC_GRAPH_KEEP_ORDER) {
-      CK(hipMemcpy(g->d_col, col, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+      CK(hipMemcpyAsync(g->d_col, col, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice, g->stream));
     } else {
       std::vector<int32_t> ordered((size_t)E);
       if (sizeof(OffT) == 8)
         fc::hub_first64((const int64_t*)off, col, V, ordered.data());
       else
         fc::hub_first32((const int32_t*)off, col, V, ordered.data());
-      CK(hipMemcpy(g->d_col, ordered.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+      CK(hipMemcpyAsync(g->d_col, ordered.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice, g->stream));
     }
   }
 
@@ -320,7 +326,8 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   g->hcap = (uint32_t)std::max<int64_t>(hubs, 1);
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off_state = 0;
-  size_t off_parent = off_state + al(sizeof(int32_t) * (size_t)V);
+  size_t off_sst = off_state + al(sizeof(int32_t) * (size_t)V);
+  size_t off_parent = off_sst + al(sizeof(unsigned long long) * (size_t)V);
   size_t off_vis = off_parent + al(sizeof(int32_t) * (size_t)V);
   size_t off_fbm = off_vis + al(bm_bytes);
   size_t off_q = off_fbm + 6 * al(bm_bytes);
@@ -329,8 +336,9 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   size_t off_ctl = off_hb + 4 * al(sizeof(int64_t) * (size_t)g->hcap);
   g->ws_bytes = off_ctl + al(sizeof(fcd::VisitCtl));
   CK(hipMalloc((void**)&g->d_ws, g->ws_bytes));
-  CK(hipMemset(g->d_ws, 0, g->ws_bytes));
+  CK(hipMemsetAsync(g->d_ws, 0, g->ws_bytes, g->stream));
   g->d_state = (int32_t*)(g->d_ws + off_state);
+  g->d_sst = (unsigned long long*)(g->d_ws + off_sst);
   g->d_parent = (int32_t*)(g->d_ws + off_parent);
   g->d_visited = (uint32_t*)(g->d_ws + off_vis);
   for (int c = 0; c < 2; ++c) {
@@ -344,7 +352,7 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   g->d_ctl = (fcd::VisitCtl*)(g->d_ws + off_ctl);
   // canonical parameters, then the copy in the batch kernel's layout
   CK(hipMalloc((void**)&g->d_mlp, 2 * sizeof(float) * fcd::kMlpParams));
-  CK(hipMemset(g->d_mlp, 0, 2 * sizeof(float) * fcd::kMlpParams));
+  CK(hipMemsetAsync(g->d_mlp, 0, 2 * sizeof(float) * fcd::kMlpParams, g->stream));
   CK(hipHostMalloc((void**)&g->h_phase, 4 * sizeof(int32_t), hipHostMallocDefault));
 
   int occ = sizeof(OffT) == 8 ? fcd::phase_occupancy<int64_t>() : fcd::phase_occupancy<int32_t>();
@@ -358,6 +366,9 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
   g->blocks = per_cu * g->cus;
   fc_status s2 = fcd::ensure_queries(g, 1024);
   if (s2 != FC_OK) return bail(s2);
+  // the uploads and clears above are ordered on the graph's stream only:
+  // finish them before any other stream (a batch lane's) can touch the graph
+  CK(hipStreamSynchronize(g->stream));
 #undef CK
   *out = g;
   return FC_OK;
@@ -397,6 +408,15 @@ extern "C" fc_status fc_graph_destroy(fc_graph* g) {
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
   return FC_OK;
+}
+
+// Copy device results to the caller on the graph's stream. The graph's
+// streams do not wait for the legacy null stream, so no plain hipMemcpy or
+// hipMemset touches its memory.
+static hipError_t read_back(void* dst, const void* src, size_t bytes, fc_graph* g) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(g->stream);
 }
 
 static void add_stats(const fcd::VisitOut& o, fc_stats* s) {
@@ -476,8 +496,8 @@ extern "C" fc_status fc_bfs(fc_graph* g, int32_t src, int32_t* levels_out, int32
   }
   const int32_t* lv = g->relabeled ? g->d_lv_out : g->d_state;
   const int32_t* pa = g->relabeled ? g->d_par_out : g->d_parent;
-  if (levels_out) FC_HIP(hipMemcpy(levels_out, lv, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
-  if (parents_out) FC_HIP(hipMemcpy(parents_out, pa, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  if (levels_out) FC_HIP(read_back(levels_out, lv, sizeof(int32_t) * (size_t)g->V, g));
+  if (parents_out) FC_HIP(read_back(parents_out, pa, sizeof(int32_t) * (size_t)g->V, g));
   return FC_OK;
 }
 
@@ -486,7 +506,7 @@ extern "C" fc_status fc_graph_levels(fc_graph* g, int32_t* levels_out) {
   if (!g || !levels_out) return fail(FC_ERR_INVALID, "fc_graph_levels: null argument");
   if (!g->levels_valid) return fail(FC_ERR_INVALID, "fc_graph_levels: no BFS result on the device");
   const int32_t* lv = g->relabeled ? g->d_lv_out : g->d_state;
-  FC_HIP(hipMemcpy(levels_out, lv, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  FC_HIP(read_back(levels_out, lv, sizeof(int32_t) * (size_t)g->V, g));
   return FC_OK;
 }
 
@@ -495,7 +515,7 @@ extern "C" fc_status fc_graph_parents(fc_graph* g, int32_t* parents_out) {
   if (!g || !parents_out) return fail(FC_ERR_INVALID, "fc_graph_parents: null argument");
   if (!g->parents_valid) return fail(FC_ERR_INVALID, "fc_graph_parents: the last BFS did not record parents");
   const int32_t* pa = g->relabeled ? g->d_par_out : g->d_parent;
-  FC_HIP(hipMemcpy(parents_out, pa, sizeof(int32_t) * (size_t)g->V, hipMemcpyDeviceToHost));
+  FC_HIP(read_back(parents_out, pa, sizeof(int32_t) * (size_t)g->V, g));
   return FC_OK;
 }
 
@@ -564,11 +584,13 @@ extern "C" fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t*
 // end at a root.
 template <typename OffT>
 __global__ void path_kernel(const OffT* __restrict__ off, const int32_t* __restrict__ col,
-                            const int32_t* __restrict__ state, const int32_t* __restrict__ parent, int32_t V,
-                            int32_t u, int32_t dist, int32_t* __restrict__ out, int32_t* __restrict__ status) {
+                            const unsigned long long* __restrict__ sst, uint32_t epoch,
+                            const int32_t* __restrict__ parent, int32_t V, int32_t u, int32_t dist,
+                            int32_t* __restrict__ out, int32_t* __restrict__ status) {
   using namespace fcd;
   const int lane = lane_id();
-  const int32_t su = state[u];
+  const unsigned long long wu = sst[u];
+  const int32_t su = (int32_t)(uint32_t)wu;
   const bool tu = (su & kColorBit) != 0;  // u on the t side
   const int32_t lu = su & kLevelMask;
   const int32_t want = dist - 1 - lu;
@@ -580,8 +602,9 @@ __global__ void path_kernel(const OffT* __restrict__ off, const int32_t* __restr
     bool hit = false;
     if (e < e1) {
       x = col[e];
-      const int32_t sx = state[x];
-      hit = sx != -1 && (((sx & kColorBit) != 0) != tu) && (sx & kLevelMask) == want;
+      const unsigned long long wx = sst[x];
+      const int32_t sx = (int32_t)(uint32_t)wx;
+      hit = (uint32_t)(wx >> 32) == epoch && (((sx & kColorBit) != 0) != tu) && (sx & kLevelMask) == want;
     }
     const uint64_t m = ballot(hit);
     if (m) {
@@ -590,7 +613,7 @@ __global__ void path_kernel(const OffT* __restrict__ off, const int32_t* __restr
     }
   }
   if (lane != 0) return;
-  if (w < 0 || want < 0) {
+  if (w < 0 || want < 0 || (uint32_t)(wu >> 32) != epoch) {
     *status = -1;
     return;
   }
@@ -645,6 +668,7 @@ extern "C" fc_status fc_stcon_path(fc_graph* g, int32_t s, int32_t t, int32_t* p
   double ms = 0;
   fc_status st = fcd::run_visits(g, fcd::kModePlainSt, &ds, &dt, 1, true, &ms);
   if (st != FC_OK) return st;
+  const uint32_t epoch = g->epoch_next - 1;  // the query's tag in the state words
   const fcd::VisitOut o = g->h_out[0];
   st = check_out(o);
   if (st != FC_OK) return st;
@@ -665,10 +689,10 @@ extern "C" fc_status fc_stcon_path(fc_graph* g, int32_t s, int32_t t, int32_t* p
   FC_HIP(hipEventRecord(g->ev0, g->stream));
   if (g->off64)
     hipLaunchKernelGGL(path_kernel<int64_t>, dim3(1), dim3(fcd::kWave), 0, g->stream, (const int64_t*)g->d_off,
-                       g->d_col, g->d_state, g->d_parent, (int32_t)g->V, o.meet, o.dist, d_path, d_status);
+                       g->d_col, g->d_sst, epoch, g->d_parent, (int32_t)g->V, o.meet, o.dist, d_path, d_status);
   else
     hipLaunchKernelGGL(path_kernel<int32_t>, dim3(1), dim3(fcd::kWave), 0, g->stream, (const int32_t*)g->d_off,
-                       g->d_col, g->d_state, g->d_parent, (int32_t)g->V, o.meet, o.dist, d_path, d_status);
+                       g->d_col, g->d_sst, epoch, g->d_parent, (int32_t)g->V, o.meet, o.dist, d_path, d_status);
   FC_HIP(hipGetLastError());
   FC_HIP(hipEventRecord(g->ev1, g->stream));
   int32_t status = 0;

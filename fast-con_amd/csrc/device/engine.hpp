@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "common.hpp"
@@ -34,6 +35,8 @@ struct fc_graph {
   char* d_ws = nullptr;
   size_t ws_bytes = 0;
   int32_t* d_state = nullptr;
+  unsigned long long* d_sst = nullptr;  // s–t state words: (epoch << 32) | colour | level
+  uint32_t epoch_next = 1;              // epoch of the next s–t query (0 = never visited)
   int32_t* d_parent = nullptr;
   uint32_t* d_visited = nullptr;
   uint32_t* d_fbm[2][3] = {};
@@ -53,7 +56,7 @@ struct fc_graph {
   // Captured hipGraph of kGraphLaunches identical phase launches.
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  double est_launches[2] = {9.0, 5.0};  // working launches per query (running estimate, per mode)
+  double est_launches[2] = {10.0, 3.0};  // working launches per query (running estimate, per mode)
   int32_t vw_fixed = 0;
   bool use_graphs = true;           // replay captured launches (FASTCON_PLAIN_GRAPHS=0: direct launches)
 
@@ -91,8 +94,33 @@ fc_status ensure_queries(fc_graph* g, int32_t n);
 fc_status run_visits(fc_graph* g, int mode, const int32_t* s, const int32_t* t, int32_t n, bool parents,
                      double* ms);
 
+// One launch of the visit kernel on the control block ctl (which carries the
+// launch parameters); launches alternate parity 0, 1, 0, ...
 template <typename OffT>
-hipError_t launch_phase(const VisitParams<OffT>& p, int blocks, hipStream_t s);
+hipError_t launch_phase(VisitCtl* ctl, int parity, int blocks, hipStream_t s);
+
+// Control block for a run of n queries: s–t queries start directly in their
+// level phase (query i has epoch epoch0 + i), a BFS starts with its reset.
+template <typename OffT>
+inline VisitCtl visit_init(const VisitParams<OffT>& prm, int mode, int32_t n, const int32_t* s, const int32_t* t,
+                           uint32_t epoch0, int vw_fixed, bool parents) {
+  VisitCtl c;
+  std::memset(&c, 0, sizeof(c));
+  std::memcpy(c.params, &prm, sizeof(prm));
+  VisitState& st = c.st[0];
+  st.phase = mode == kModeBfs ? kPhaseReset : kPhaseLevel;
+  st.qi = 0;
+  st.src = s[0];
+  st.dst = t[0];
+  st.nsrc = st.ndst = -1;
+  st.epoch = epoch0;
+  c.best[0] = c.best[1] = ~0ull;
+  c.nq = n;
+  c.mode = mode;
+  c.vw_fixed = vw_fixed;
+  c.record_parents = parents ? 1 : 0;
+  return c;
+}
 template <typename OffT>
 int phase_occupancy();
 

@@ -18,9 +18,10 @@
 // device current) and keeps it; kernels get that address as an argument.
 // Every call on one GPU shares that GPU's array, so calls that overlap on one
 // GPU (from several host threads) race: serialise them. Each call writes the
-// control block and the query list before its first launch, and the reset
-// phase of each query rewrites the state of all V vertices, so nothing left
-// by an earlier call (a larger graph, more queries, an aborted run) is read.
+// control block and the query list before its first launch, and every query
+// tags the state words it writes with an epoch of its own (a word of another
+// epoch counts as unvisited), so nothing left by an earlier call (a larger
+// graph, more queries, an aborted run) is read as this call's.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -44,10 +45,10 @@ constexpr int32_t kStaticQueries = 1024;  // queries per pass of the phase machi
 constexpr int kMaxDevices = 64;
 
 constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
-// Layout of the array: state, four frontier queues, four hub queues (vertex
-// and edge base), the control block, the query lists and the results.
-constexpr size_t kOffState = 0;
-constexpr size_t kOffQ = kOffState + al256(sizeof(int32_t) * kStaticMaxV);
+// Layout of the array: state words, four frontier queues, four hub queues
+// (vertex and edge base), the control block, the query lists and the results.
+constexpr size_t kOffSst = 0;
+constexpr size_t kOffQ = kOffSst + al256(sizeof(unsigned long long) * kStaticMaxV);
 constexpr size_t kOffHq = kOffQ + 4 * al256(sizeof(int32_t) * kStaticMaxV);
 constexpr size_t kOffHb = kOffHq + 4 * al256(sizeof(int32_t) * kStaticHubCap);
 constexpr size_t kOffCtl = kOffHb + 4 * al256(sizeof(int64_t) * kStaticHubCap);
@@ -58,7 +59,7 @@ constexpr size_t kStaticBytes = kOffOut + al256(sizeof(VisitOut) * kStaticQuerie
 
 }  // namespace
 
-// The workspace of fc_stcon_many_static (about 23 MB per device).
+// The workspace of fc_stcon_many_static (about 27 MB per device).
 static __device__ __align__(256) unsigned char scratch[kStaticBytes];
 
 namespace {
@@ -66,7 +67,8 @@ namespace {
 struct DeviceSlot {
   unsigned char* ptr = nullptr;  // device address of `scratch` on this device
   int blocks = 0;
-  double est_launches = 5.0;     // working launches per query (running estimate)
+  double est_launches = 3.0;     // working launches per query (running estimate)
+  uint32_t epoch_next = 1;       // epoch of the next query (0 = never visited)
 };
 DeviceSlot g_slots[kMaxDevices];
 std::mutex g_slots_mu;
@@ -103,9 +105,9 @@ VisitParams<int32_t> static_params(unsigned char* base, const int32_t* off, cons
   p.V = V;
   p.hub_threshold = kStaticHub;
   p.E = E;
-  p.state = (int32_t*)(base + kOffState);
-  // The plain s–t mode keeps no parents and no bitmaps (visit.hip): those
-  // pointers stay null and are never dereferenced in this mode.
+  p.sst = (unsigned long long*)(base + kOffSst);
+  // The plain s–t mode keeps no levels, parents or bitmaps (visit.hip):
+  // those pointers stay null and are never dereferenced in this mode.
   for (int c = 0; c < 2; ++c)
     for (int k = 0; k < 2; ++k) {
       const int i = c * 2 + k;
@@ -125,29 +127,28 @@ VisitParams<int32_t> static_params(unsigned char* base, const int32_t* off, cons
 }
 
 // One pass of the phase machine over n <= kStaticQueries queries already in
-// the array; the results are left in p.out. Launches go to the null stream;
-// the phase word is polled with synchronous copies into host stack memory.
-fc_status static_pass(DeviceSlot& slot, const VisitParams<int32_t>& p, int32_t n, int32_t s0, int32_t t0) {
-  VisitCtl init;
-  std::memset(&init, 0, sizeof(init));
-  init.phase = kPhaseReset;
-  init.nq = n;
-  init.mode = kModePlainSt;
-  init.src = s0;
-  init.dst = t0;
-  init.best = ~0ull;
+// the array; the results are left in p.out. Launches go to the null stream
+// (parity 0, 1, 0, ...); the phase word is polled with synchronous copies
+// into host stack memory.
+fc_status static_pass(DeviceSlot& slot, const VisitParams<int32_t>& p, int32_t n, const int32_t* s, const int32_t* t) {
+  if ((uint64_t)slot.epoch_next + (uint64_t)n >= 0xFFFFFFF0ull) {
+    // the epoch counter would wrap: clear the state words first
+    FC_HIP(hipMemset(p.sst, 0, sizeof(unsigned long long) * kStaticMaxV));
+    slot.epoch_next = 1;
+  }
+  const VisitCtl init = visit_init(p, kModePlainSt, n, s, t, slot.epoch_next, 0, false);
+  slot.epoch_next += (uint32_t)n;
   FC_HIP(hipMemcpy(p.ctl, &init, sizeof(init), hipMemcpyHostToDevice));
-  int64_t expected = (int64_t)((double)n * slot.est_launches) + 1;
+  int64_t expected = (int64_t)((double)n * slot.est_launches) + 2;
   int64_t launched = 0;
   for (int round = 0;; ++round) {
     int64_t want = std::min<int64_t>(std::max<int64_t>(expected - launched, 1), kPollLaunches);
-    for (int64_t i = 0; i < want; ++i) {
-      hipError_t e = launch_phase<int32_t>(p, slot.blocks, nullptr);
+    for (int64_t i = 0; i < want; ++i, ++launched) {
+      hipError_t e = launch_phase<int32_t>(p.ctl, (int)(launched & 1), slot.blocks, nullptr);
       if (e != hipSuccess) return hip_fail(e, "phase kernel launch");
     }
-    launched += want;
     int32_t word[2] = {0, 0};
-    FC_HIP(hipMemcpy(word, p.ctl, sizeof(word), hipMemcpyDeviceToHost));
+    FC_HIP(hipMemcpy(word, &p.ctl->st[launched & 1], sizeof(word), hipMemcpyDeviceToHost));
     if (word[0] == kPhaseIdle) break;
     if (launched > (int64_t)n * ((int64_t)p.V + 4) + 64)
       return fail(FC_ERR_CAPACITY, "phase machine did not finish after %lld launches", (long long)launched);
@@ -202,7 +203,7 @@ extern "C" fc_status fc_stcon_many_static(const int32_t* d_off, const int32_t* d
     if (k == 0) break;
     FC_HIP(hipMemcpy((void*)p.qs, hs, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice));
     FC_HIP(hipMemcpy((void*)p.qt, ht, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice));
-    st = static_pass(*slot, p, k, hs[0], ht[0]);
+    st = static_pass(*slot, p, k, hs, ht);
     if (st != FC_OK) return st;
     double wl = 0;
     for (int32_t j0 = 0; j0 < k; j0 += 64) {

@@ -1,19 +1,34 @@
 // Visit engine for BFS and s–t connectivity on gfx950: one kernel launch per
-// phase (the reset of a query, or one level of it), driven by a device-side
-// phase machine in VisitCtl.
+// level, driven by a device-side phase machine in VisitCtl.
 //
 // Why launches and not a persistent kernel: on MI355X a grid-wide software
 // barrier with the agent-scope release/acquire that cross-XCD visibility
 // needs costs 16-35 us, a cooperative launch ~20 us, while a back-to-back
 // ordinary launch costs ~2.4 us (1.6 us inside a hipGraph) and its boundary
-// already orders every block's writes. So each launch does one phase with all
-// blocks, and the last block to finish (an atomic ticket, no waiting) takes
-// the decisions for the next launch: direction, virtual-warp width and side,
-// termination, result write-back, and the move to the next query. The host
-// replays a captured hipGraph of identical launches; launches after the last
-// query return at once. This replaces the reference's host-driven level loop
-// with a 16-byte readback per level (BFS-4K/src/device/BFS_WorkEfficient.cu:81-118)
+// already orders every block's writes. So each launch does one level with
+// all blocks. This replaces the reference's host-driven level loop with a
+// 16-byte readback per level (BFS-4K/src/device/BFS_WorkEfficient.cu:81-118)
 // and ST-CON's ~12 host round trips per level (st-conn/make_stcon.c:968-1187).
+//
+// No block votes and no block finalises. Every block loads the 1 KB control
+// block at the start of a launch (one 16-byte load per lane of one wave) and
+// derives the same step from it: whether the query is finished (met, empty
+// frontier, error), the direction and the virtual-warp width. Block 0 writes
+// the machine state of the next launch into the other half of a buffer
+// indexed by launch parity, which no block of this launch reads, and writes
+// the results of a finished query. A level's critical path is therefore the
+// level's own memory traffic, with no atomic ticket and no serial chain of
+// loads by a finalising block after it. The price is that a query's end is
+// noticed at the start of the launch after its last level: an s–t query
+// spends that launch expanding the next query's roots, so n queries of d
+// levels cost n*d + 1 launches.
+//
+// s–t state words are 64-bit: (epoch << 32) | colour | level, and a word of
+// another epoch is an unvisited vertex. Each query has its own epoch, so no
+// query clears the state array: the next query starts in the same launch
+// that sees the last one finish. Its roots are never claimed through the
+// array; a probe that reaches the other colour's root proposes a meeting at
+// the root's level 0.
 //
 // Frontier expansion (per colour, per level):
 //  * top-down: normal-queue vertices are expanded by virtual warps of VW lanes
@@ -24,19 +39,19 @@
 //    adjacency is cut into chunks spread over the whole grid: the device-side
 //    work queue that replaces the dynamic-parallelism child grids
 //    (BFS-4K/src/device/WorkEfficientKernel/BFS_WE_Dynamic.cu:4-50).
-//  * bottom-up: each wave owns 64 vertices; unvisited ones look for a parent
-//    in the current frontier bitmap (direction-optimising BFS, Beamer et al. SC'12).
-// Frontier slots are reserved with one atomic per wave (64-bit ballot + mbcnt),
-// the in-house replacement of FrontierWrite::Write (BFS-4K/src/device/Util/GlobalWrite.cu:49-105)
+//  * bottom-up (BFS): each wave owns 64 vertices; unvisited ones look for a
+//    parent in the current frontier bitmap (direction-optimising BFS,
+//    Beamer et al. SC'12).
+// Frontier slots are reserved through LDS stages, one atomic per block, the
+// in-house replacement of FrontierWrite::Write (BFS-4K/src/device/Util/GlobalWrite.cu:49-105)
 // and of ST-CON's Thrust exclusive_scan / remove_copy compaction
 // (st-conn/mythrustlib.cu:25-34,175-185), which cost O(V) per level.
 //
-// s–t: the state word is colour | level; a vertex is claimed with ONE atomicCAS
-// (ST-CON k_stcon_binary_mask_unique_large, st-conn/cudakernel/stcon_kernel.cu:128-196).
-// A CAS (or read) that shows the other colour is a meeting edge (v,u) and
-// proposes the path length level(v)+1+level(u); the visit ends after the level
-// in which the first proposal appears, and the minimum proposal is the exact
-// distance.
+// s–t claims take ONE atomicCAS per vertex (ST-CON k_stcon_binary_mask_unique_large,
+// st-conn/cudakernel/stcon_kernel.cu:128-196). A CAS (or read) that shows the
+// other colour is a meeting edge (v,u) and proposes the path length
+// level(v)+1+level(u); the query ends after the level in which the first
+// proposal appears, and the minimum proposal is the exact distance.
 #include "visit.hpp"
 #include "hubstage.hpp"
 
@@ -61,7 +76,7 @@ __device__ __forceinline__ bool test_bit(const uint32_t* bm, int32_t v) {
 
 // Next-frontier appender of one wave. Normal-queue entries are staged in
 // this wave's LDS buffer; a full buffer is published by the wave with one
-// atomic, and at the end of the phase the whole block publishes its buffers,
+// atomic, and at the end of the level the whole block publishes its buffers,
 // degree sums and best meeting candidate with one atomic each. A single
 // frontier counter hit by every wave-iteration was the bottleneck of large
 // levels (a contended word sustains only tens of millions of atomics/s).
@@ -74,7 +89,10 @@ struct Appender {
   int nxt_par;
   Ring* nxt;
   uint32_t* nfbm;
-  bool bitmaps;                 // keep the visited and frontier bitmaps (bottom-up levels may follow)
+  unsigned long long* best_out;  // the query's best meeting candidate (device)
+  bool bitmaps;                 // keep the visited and frontier bitmaps (BFS: bottom-up levels may follow)
+  uint32_t epoch;               // s–t: tag of the current query
+  int32_t root_own, root_other; // s–t: this colour's root and the other colour's
   int32_t* buf;                 // this wave's LDS stage, kStageCap entries
   uint32_t cnt;                 // staged entries (wave-uniform)
   uint64_t mf;                  // per-lane degree sum of claimed vertices
@@ -135,57 +153,98 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
   return v;
 }
 
-// Block-collective (all threads, uniform control flow): publish every wave's
-// staged entries with one atomic, and the degree sum and best candidate.
+// Block-collective publication of both colours' staged entries (all threads,
+// uniform control flow, at the end of the level). One barrier publishes every
+// wave's counts; then four waves each take one list with one atomic, at the
+// same time: wave 0 colour 0's normal list (with its degree sum and best
+// meeting candidate), wave 1 colour 1's, waves 2 and 3 the two hub lists;
+// after a second barrier every wave writes its entries.
 struct BlockShared {
-  uint32_t cnt[kWavesPerBlock];
-  unsigned long long mf[kWavesPerBlock];
-  unsigned long long best[kWavesPerBlock];
-  uint32_t base;
-  HubShared hub;
+  uint32_t cnt[2][kWavesPerBlock];
+  unsigned long long mf[2][kWavesPerBlock];
+  unsigned long long best[2][kWavesPerBlock];
+  uint32_t hcnt[2][kWavesPerBlock];
+  unsigned long long hedges[2][kWavesPerBlock];
+  uint32_t base[2];
+  unsigned long long hbase[2];
 };
+static_assert(kWavesPerBlock >= 4, "one publishing wave per list");
 
 template <typename OffT>
-__device__ __forceinline__ void block_flush(Appender<OffT>& a, BlockShared& sh) {
+__device__ __forceinline__ void block_flush(Appender<OffT>& a, int ncol, BlockShared& sh, Ring* nxt0, Ring* nxt1) {
   const int w = (int)(threadIdx.x >> 6);
-  const unsigned long long mf = (unsigned long long)wave_sum_u64(a.mf);
-  const unsigned long long best = wave_min_u64(a.best);
-  if (lane_id() == 0) {
-    sh.cnt[w] = a.cnt;
-    sh.mf[w] = mf;
-    sh.best[w] = best;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t tot = 0;
-    unsigned long long mft = 0, bt = ~0ull;
-    for (int i = 0; i < kWavesPerBlock; ++i) {
-      const uint32_t c = sh.cnt[i];
-      sh.cnt[i] = tot;
-      tot += c;
-      mft += sh.mf[i];
-      bt = sh.best[i] < bt ? sh.best[i] : bt;
+  const int lane = lane_id();
+  const int own = a.c;  // this wave staged entries of its own colour only
+  {
+    const unsigned long long mf = (unsigned long long)wave_sum_u64(a.mf);
+    const unsigned long long best = wave_min_u64(a.best);
+    const unsigned long long hd = wave_sum_u64((uint32_t)lane < a.h.cnt ? (unsigned long long)a.h.d[lane] : 0ull);
+    if (lane == 0) {
+      sh.cnt[own][w] = a.cnt;
+      sh.mf[own][w] = mf;
+      sh.best[own][w] = best;
+      sh.hcnt[own][w] = a.h.cnt;
+      sh.hedges[own][w] = hd;
+      sh.cnt[own ^ 1][w] = 0;
+      sh.mf[own ^ 1][w] = 0;
+      sh.best[own ^ 1][w] = ~0ull;
+      sh.hcnt[own ^ 1][w] = 0;
+      sh.hedges[own ^ 1][w] = 0;
     }
-    sh.base = tot ? atomicAdd(&a.nxt->n, tot) : 0u;
-    if (mft) atomicAdd((unsigned long long*)&a.nxt->mf, mft);
-    if (bt != ~0ull && bt < ld_agent(&a.p->ctl->best)) atomicMin(&a.p->ctl->best, bt);
   }
   __syncthreads();
-  const uint32_t base = sh.base + sh.cnt[w];
-  int32_t* q = a.p->q[a.c][a.nxt_par];
-  for (uint32_t i = (uint32_t)lane_id(); i < a.cnt; i += kWave) {
+  const int pc = w & 1;  // the list this wave publishes: colour pc, normal (w < 2) or hub (w >= 2)
+  if (lane == 0 && w < 4 && pc < ncol) {
+    Ring* nxt = pc ? nxt1 : nxt0;
+    if (w < 2) {
+      uint32_t tot = 0;
+      unsigned long long mft = 0, bt = ~0ull;
+      for (int i = 0; i < kWavesPerBlock; ++i) {
+        const uint32_t k = sh.cnt[pc][i];
+        sh.cnt[pc][i] = tot;
+        tot += k;
+        mft += sh.mf[pc][i];
+        bt = sh.best[pc][i] < bt ? sh.best[pc][i] : bt;
+      }
+      sh.base[pc] = tot ? atomicAdd(&nxt->n, tot) : 0u;
+      if (mft) atomicAdd((unsigned long long*)&nxt->mf, mft);
+      if (bt != ~0ull) atomicMin(a.best_out, bt);
+    } else {
+      uint32_t k = 0;
+      unsigned long long e = 0;
+      for (int i = 0; i < kWavesPerBlock; ++i) {
+        const uint32_t ki = sh.hcnt[pc][i];
+        const unsigned long long ei = sh.hedges[pc][i];
+        sh.hcnt[pc][i] = k;
+        sh.hedges[pc][i] = e;
+        k += ki;
+        e += ei;
+      }
+      sh.hbase[pc] = k ? atomicAdd((unsigned long long*)&nxt->hub, ((unsigned long long)k << kHubEdgeBits) | e)
+                       : 0ull;
+    }
+  }
+  __syncthreads();
+  const uint32_t base = sh.base[own] + sh.cnt[own][w];
+  int32_t* q = a.p->q[own][a.nxt_par];
+  for (uint32_t i = (uint32_t)lane; i < a.cnt; i += kWave) {
     const uint32_t slot = base + i;
     if (slot < a.p->qcap)
       q[slot] = a.buf[i];
     else
       atomicOr(&a.p->ctl->err, 1u);
   }
-  a.cnt = 0;
-  a.mf = 0;
-  a.best = ~0ull;
-  __syncthreads();
-  hub_block_flush(a.h, (unsigned long long*)&a.nxt->hub, a.p->hq[a.c][a.nxt_par], a.p->hb[a.c][a.nxt_par], a.p->hcap,
-                  &a.p->ctl->err, sh.hub);
+  if (a.h.cnt) {
+    const unsigned long long hb = sh.hbase[own];
+    hub_write(a.h, (uint32_t)(hb >> kHubEdgeBits) + sh.hcnt[own][w], (hb & kHubEdgeMask) + sh.hedges[own][w],
+              a.p->hq[own][a.nxt_par], a.p->hb[own][a.nxt_par], a.p->hcap, &a.p->ctl->err);
+  }
+}
+
+template <typename OffT>
+__device__ __forceinline__ void propose(Appender<OffT>& a, int32_t dist, int32_t u) {
+  const unsigned long long cand = ((unsigned long long)dist << 32) | (uint32_t)u;
+  a.best = cand < a.best ? cand : a.best;
 }
 
 // Probe edge (v -> u) for colour c whose frontier is at level lvl. Returns
@@ -207,22 +266,30 @@ __device__ __forceinline__ bool probe_td(Appender<OffT>& a, int32_t* parent, boo
     if (parent) parent[u] = v;
     return true;
   }
-  const int c = a.c;
-  const int32_t mine = (c ? kColorBit : 0) | (lvl + 1);
-  int32_t cur = p.state[u];
-  if (cur == -1) {
-    cur = atomicCAS(&p.state[u], -1, mine);
-    if (cur == -1) {
-      if (a.bitmaps) atomicOr(p.visited + (u >> 5), 1u << (u & 31));
+  if (u == a.root_own) return false;
+  if (u == a.root_other) {
+    propose(a, lvl + 1, u);
+    return false;
+  }
+  const unsigned long long mine = st_word(a.epoch, a.c, lvl + 1);
+  unsigned long long cur = p.sst[u];
+  // A word of another epoch is unvisited. The plain load may be stale, so a
+  // failed CAS retries with the value it returns; within a launch the word
+  // only changes to this epoch, so the second attempt settles it.
+  for (int k = 0; k < 3 && (uint32_t)(cur >> 32) != a.epoch; ++k) {
+    const unsigned long long old = atomicCAS(&p.sst[u], cur, mine);
+    if (old == cur) {
       if (parent) parent[u] = v;
       return true;
     }
+    cur = old;
   }
-  if (((cur & kColorBit) != 0) != (c != 0)) {
-    const unsigned long long cand =
-        ((unsigned long long)(lvl + 1 + (cur & kLevelMask)) << 32) | (uint32_t)u;
-    a.best = cand < a.best ? cand : a.best;
+  if ((uint32_t)(cur >> 32) != a.epoch) {
+    atomicOr(&p.ctl->err, 4u);
+    return false;
   }
+  const int32_t w = (int32_t)(uint32_t)cur;
+  if (((w & kColorBit) != 0) != (a.c != 0)) propose(a, lvl + 1 + (w & kLevelMask), u);
   return false;
 }
 
@@ -232,8 +299,8 @@ __device__ __forceinline__ int64_t degree_of(const VisitParams<OffT>& p, int32_t
 }
 
 template <typename OffT, int VW>
-__device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par, uint32_t n, int gwave,
-                          int nwaves) {
+__device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par,
+                                          uint32_t n, int gwave, int nwaves) {
   const VisitParams<OffT>& p = *a.p;
   constexpr int G = kWave / VW;
   const int lane = lane_id();
@@ -278,8 +345,8 @@ __device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bo
 constexpr int kHubChunkMax = 256;
 
 template <typename OffT>
-__device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par, uint64_t hubpk, int gwave,
-                        int nwaves) {
+__device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int cur_par,
+                                        uint64_t hubpk, int gwave, int nwaves) {
   const VisitParams<OffT>& p = *a.p;
   uint32_t nh = (uint32_t)(hubpk >> kHubEdgeBits);
   if (nh > p.hcap) nh = p.hcap;
@@ -333,63 +400,72 @@ __device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool
   }
 }
 
+// Level 0 of an s–t query: the root's adjacency [b, e) cut into 64-edge
+// chunks over the waves of the grid (one edge per lane), whatever its degree.
+template <typename OffT>
+__device__ __forceinline__ void td_root(Appender<OffT>& a, int32_t* parent, int32_t r, OffT b, OffT e, int gwave,
+                                        int nwaves) {
+  const VisitParams<OffT>& p = *a.p;
+  for (OffT k = b + (OffT)gwave * kWave; k < e; k += (OffT)nwaves * kWave) {
+    const OffT j = k + (OffT)lane_id();
+    const bool act = j < e;
+    const int32_t u = act ? p.col[j] : 0;
+    const bool cl = probe_td(a, parent, true, 0, r, u, act);
+    push_next(a, cl, u, cl ? degree_of(p, u) : 0, false);
+  }
+}
+
 // Candidates of degree <= kBuLane scan their own list; longer lists are
 // scanned by the whole wave, 64 edges at a time.
 constexpr int kBuLane = 128;
 
-// Bottom-up step for colour c: every unvisited vertex looks for a neighbour in
-// c's current frontier bitmap. In s–t modes the vertices of the other colour's
-// current frontier are scanned too, so that a frontier-to-frontier edge is
-// still seen as a meeting edge.
 // Per-chunk loads of a bottom-up step (64 consecutive vertices of one wave):
-// the candidate flags and the offset pair. They are issued for the next
+// the unvisited flags and the offset pair. They are issued for the next
 // chunk before the current chunk's columns are scanned, so a chunk costs
 // about two dependent memory round trips instead of three.
 template <typename OffT>
 struct BuChunk {
-  bool unvis, oth;
+  bool unvis;
   OffT b, e;
 };
 
 template <typename OffT>
-__device__ __forceinline__ BuChunk<OffT> bu_load(const VisitParams<OffT>& p, const uint32_t* ofbm, uint32_t w) {
+__device__ __forceinline__ BuChunk<OffT> bu_load(const VisitParams<OffT>& p, uint32_t w) {
   const int lane = lane_id();
   const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
-  const bool inr = v < p.V;
   const uint32_t wi = 2u * w + ((uint32_t)lane >> 5);
   BuChunk<OffT> c;
-  c.unvis = inr && !((p.visited[wi] >> (lane & 31)) & 1u);
-  c.oth = inr && ofbm && ((ofbm[wi] >> (lane & 31)) & 1u);
+  c.unvis = v < p.V && !((p.visited[wi] >> (lane & 31)) & 1u);
   c.b = c.e = 0;
-  if (c.unvis || c.oth) {
+  if (c.unvis) {
     c.b = p.off[v];
     c.e = p.off[v + 1];
   }
   return c;
 }
 
+// Bottom-up BFS step: every unvisited vertex looks for a neighbour in the
+// current frontier bitmap.
 template <typename OffT>
-__device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool st, int lvl, const uint32_t* cfbm,
-                        const uint32_t* ofbm, int gwave, int nwaves) {
+__device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int lvl, const uint32_t* cfbm, int gwave,
+                                        int nwaves) {
   const VisitParams<OffT>& p = *a.p;
-  const int c = a.c;
   uint32_t* nfbm = a.nfbm;
   const int lane = lane_id();
   // Isolated vertices (the tail of a degree-ordered numbering) can have no
   // parent: their chunks are skipped (43% of the vertices of R-MAT s22).
   const uint32_t nw64 = ((uint32_t)p.v_active + 63u) >> 6;
-  const int32_t mine = st ? ((c ? kColorBit : 0) | (lvl + 1)) : lvl + 1;
+  const int32_t mine = lvl + 1;
   uint32_t w = (uint32_t)gwave;
   if (w >= nw64) return;
-  BuChunk<OffT> cur = bu_load(p, ofbm, w);
+  BuChunk<OffT> cur = bu_load(p, w);
   for (;;) {
     const uint32_t wn = w + (uint32_t)nwaves;
     const bool more = wn < nw64;
     BuChunk<OffT> nx;
-    if (more) nx = bu_load(p, ofbm, wn);
+    if (more) nx = bu_load(p, wn);
     const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
-    const bool unvis = cur.unvis, oth = cur.oth;
-    const bool cand = unvis || oth;
+    const bool cand = cur.unvis;
     const OffT b = cur.b, e = cur.e;
     const bool small = cand && (e - b) <= (OffT)kBuLane;
     int32_t par = -1;
@@ -437,16 +513,10 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool
       }
       if (lane == l) par = fp;
     }
-    const bool found = par >= 0;
-    bool claimed = false;
-    if (found && unvis) {
-      claimed = true;
+    const bool claimed = cand && par >= 0;
+    if (claimed) {
       p.state[v] = mine;
       if (parent) parent[v] = par;
-    } else if (found && oth) {
-      const int32_t sv = p.state[v];
-      const unsigned long long cnd = ((unsigned long long)(lvl + 1 + (sv & kLevelMask)) << 32) | (uint32_t)v;
-      a.best = cnd < a.best ? cnd : a.best;
     }
     // Bitmap words of these 64 vertices belong to this wave during a
     // bottom-up level, so they are updated without atomics (and the
@@ -468,8 +538,8 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, bool
 }
 
 template <typename OffT>
-__device__ __forceinline__ void td_dispatch(Appender<OffT>& a, int32_t* parent, bool st, int vw, int lvl, int cur_par, uint32_t n,
-                            int gwave, int nwaves) {
+__device__ __forceinline__ void td_dispatch(Appender<OffT>& a, int32_t* parent, bool st, int vw, int lvl, int cur_par,
+                                            uint32_t n, int gwave, int nwaves) {
   switch (vw) {
     case 4: td_normal<OffT, 4>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
     case 8: td_normal<OffT, 8>(a, parent, st, lvl, cur_par, n, gwave, nwaves); break;
@@ -493,272 +563,293 @@ __device__ __forceinline__ int degree_width(uint32_t n, uint64_t mf_normal) {
 constexpr double kAlpha = 14.0;
 constexpr double kBeta = 24.0;
 
-// Decision for the next level launch (one thread of the finalising block).
-__device__ void decide(VisitCtl* ctl, int64_t V, int64_t E) {
-  Decision d;
-  d.ex[0] = d.ex[1] = 0;
-  d.strat[0] = d.strat[1] = kTopDown;
-  d.vw[0] = d.vw[1] = 8;
-  uint32_t cn[2];
-  uint64_t cmf[2], chub[2];
-  for (int c = 0; c < 2; ++c) {
-    const Ring* R = &ctl->ring[c][ctl->lvl[c] % 3];
-    cn[c] = ld_agent(&R->n);
-    cmf[c] = ld_agent(&R->mf);
-    chub[c] = ld_agent(&R->hub);
-  }
-  const double mu = (double)E - (double)ctl->seen_edges;
-  const int mode = ctl->mode;
-  auto width = [&](int c) {
-    return ctl->vw_fixed > 0 ? ctl->vw_fixed : degree_width(cn[c], cmf[c] - (chub[c] & kHubEdgeMask));
-  };
-  if (mode == kModeBfs) {
-    d.ex[0] = 1;
-    const uint64_t nf = (uint64_t)cn[0] + (chub[0] >> kHubEdgeBits);
-    if (!ctl->bu_mode) {
-      if ((double)cmf[0] > mu / kAlpha) ctl->bu_mode = 1;
-    } else if ((double)nf * kBeta < (double)V) {
-      ctl->bu_mode = 0;
-    }
-    d.strat[0] = ctl->bu_mode ? kBottomUp : kTopDown;
-    d.vw[0] = width(0);
-  } else {
-    d.ex[0] = d.ex[1] = 1;
-    d.vw[0] = width(0);
-    d.vw[1] = width(1);
-  }
-  ctl->dec = d;
-}
+__device__ __forceinline__ uint32_t ring_size(const Ring& r) { return r.n + (uint32_t)(r.hub >> kHubEdgeBits); }
 
-template <typename OffT>
-__device__ void finalize_reset(const VisitParams<OffT>& p) {
-  VisitCtl* ctl = p.ctl;
-  const bool st = ctl->mode != kModeBfs;
-  uint64_t seen = 0;
-  for (int c = 0; c < 2; ++c)
-    for (int k = 0; k < 3; ++k) ctl->ring[c][k] = Ring{0u, 0u, 0ull, 0ull};
-  for (int c = 0; c < (st ? 2 : 1); ++c) {
-    const int32_t r = c ? ctl->dst : ctl->src;
-    const int64_t d = (int64_t)(p.off[r + 1] - p.off[r]);
-    Ring& R = ctl->ring[c][0];
-    if (d >= p.hub_threshold) {
-      p.hq[c][0][0] = r;
-      p.hb[c][0][0] = 0;
-      R.hub = (1ull << kHubEdgeBits) | (uint64_t)d;
-    } else {
-      p.q[c][0][0] = r;
-      R.n = 1u;
-    }
-    R.mf = (uint64_t)d;
-    seen += (uint64_t)d;
-  }
-  ctl->lvl[0] = ctl->lvl[1] = 0;
-  ctl->steps = ctl->td = ctl->bu = ctl->maxf = 0;
-  ctl->launches = 1;
-  ctl->bu_mode = 0;
-  ctl->best = ~0ull;
-  ctl->err = 0u;
-  ctl->seen_edges = seen;
-  ctl->visited_v = st ? 2 : 1;
-  ctl->visited_e = (int64_t)seen;
-  ctl->expv = ctl->expe = 0;
-  decide(ctl, p.V, p.E);
-  ctl->phase = kPhaseLevel;
-}
+__device__ __forceinline__ void zero_ring(Ring* r) { *r = Ring{0u, 0u, 0ull, 0ull}; }
 
+// BFS reset (one launch): clear the bitmaps (and, on the caller's numbering,
+// the levels and parents), and block 0 seeds the root.
 template <typename OffT>
-__device__ void finalize_level(const VisitParams<OffT>& p, const Decision& d) {
-  VisitCtl* ctl = p.ctl;
-  const bool st = ctl->mode != kModeBfs;
-  ctl->steps++;
-  ctl->launches++;
-  bool empty = false;
-  for (int c = 0; c < 2; ++c) {
-    if (!d.ex[c]) continue;
-    const Ring* cur = &ctl->ring[c][ctl->lvl[c] % 3];
-    const uint32_t nf = cur->n + (uint32_t)(cur->hub >> kHubEdgeBits);
-    ctl->expv += nf;
-    ctl->expe += (int64_t)cur->mf;
-    if ((int32_t)nf > ctl->maxf) ctl->maxf = (int32_t)nf;
-    if (d.strat[c] == kTopDown)
-      ctl->td++;
-    else
-      ctl->bu++;
-    ctl->lvl[c]++;
-    const Ring* R = &ctl->ring[c][ctl->lvl[c] % 3];
-    const uint64_t nmf = ld_agent(&R->mf);
-    const uint32_t nn = ld_agent(&R->n) + (uint32_t)(ld_agent(&R->hub) >> kHubEdgeBits);
-    ctl->seen_edges += nmf;
-    ctl->visited_e += (int64_t)nmf;
-    ctl->visited_v += nn;
-    if (nn == 0) empty = true;
-  }
-  const unsigned long long best = ld_agent(&ctl->best);
-  const uint32_t err = ld_agent(&ctl->err);
-  const bool met = st && best != ~0ull;
-  const bool runaway = ctl->steps > p.V + 2;
-  if (met || empty || runaway || err) {
-    VisitOut o;
-    o.status = (err || runaway) ? FC_CAPACITY_STATUS : 0;
-    o.levels = ctl->steps;
-    o.depth = met ? ctl->steps : 0;
-    o.dist = met ? (int32_t)(best >> 32) : -1;
-    o.meet = met ? (int32_t)(uint32_t)best : -1;
-    o.topdown = ctl->td;
-    o.bottomup = ctl->bu;
-    o.max_frontier = ctl->maxf;
-    o.visited_vertices = ctl->visited_v;
-    o.visited_edges = ctl->visited_e;
-    o.expanded_vertices = ctl->expv;
-    o.expanded_edges = ctl->expe;
-    o.lvl_end[0] = ctl->lvl[0];
-    o.lvl_end[1] = ctl->lvl[1];
-    o.err = (int32_t)err;
-    o.launches = ctl->launches;
-    p.out[ctl->qi] = o;
-    ctl->qi++;
-    if (ctl->qi < ctl->nq) {
-      ctl->src = p.qs[ctl->qi];
-      ctl->dst = p.qt[ctl->qi];
-      ctl->phase = kPhaseReset;
-    } else {
-      ctl->phase = kPhaseIdle;
-    }
-    return;
-  }
-  for (int c = 0; c < 2; ++c) ctl->ring[c][(ctl->lvl[c] + 1) % 3] = Ring{0u, 0u, 0ull, 0ull};
-  decide(ctl, p.V, p.E);
-}
-
-template <typename OffT>
-__global__ __launch_bounds__(kBlock) void phase_kernel(VisitParams<OffT> p) {
-  VisitCtl* ctl = p.ctl;
-  const int phase = ctl->phase;
-  if (phase == kPhaseIdle) return;
-  const uint32_t nb = gridDim.x;
-  const int gwave = (int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-  const int nwaves = (int)(nb * kWavesPerBlock);
+__device__ void bfs_reset(const VisitParams<OffT>& p, const VisitState& S, VisitState* nxt, int32_t* parent) {
   const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t nthr = (int64_t)nb * kBlock;
+  const int64_t nthr = (int64_t)gridDim.x * kBlock;
   const uint32_t nwords = (((uint32_t)p.V + 63u) >> 6) * 2u;
-  const bool st = ctl->mode != kModeBfs;
-  // The plain s–t visit runs top-down only: it claims through state alone
-  // and keeps neither the visited nor the frontier bitmaps.
-  const bool bitmaps = ctl->mode != kModePlainSt;
-  int32_t* const parent = ctl->record_parents ? p.parent : nullptr;
-  __shared__ int s_last;
+  const int32_t src = S.src;
+  if (!p.lazy_state) {
+    for (int64_t v = gtid; v < p.V; v += nthr) {
+      p.state[v] = v == src ? 0 : -1;
+      if (parent) parent[v] = v == src ? (int32_t)v : -1;
+    }
+  }
+  for (int64_t w = gtid; w < nwords; w += nthr) {
+    const uint32_t bs = ((uint32_t)src >> 5) == (uint32_t)w ? 1u << (src & 31) : 0u;
+    p.visited[w] = bs;
+    p.fbm[0][0][w] = bs;
+    p.fbm[0][1][w] = 0u;
+    p.fbm[0][2][w] = 0u;
+  }
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  VisitCtl* ctl = p.ctl;
+  if (p.lazy_state) {
+    // renumbered graph: only reached vertices' state and parent are written
+    // (the visited bitmap marks them), so only the root's here
+    p.state[src] = 0;
+    if (parent) parent[src] = src;
+  }
+  const int64_t d = (int64_t)(p.off[src + 1] - p.off[src]);
+  Ring r0{0u, 0u, 0ull, (uint64_t)d};
+  if (d >= p.hub_threshold) {
+    p.hq[0][0][0] = src;
+    p.hb[0][0][0] = 0;
+    r0.hub = (1ull << kHubEdgeBits) | (uint64_t)d;
+  } else {
+    p.q[0][0][0] = src;
+    r0.n = 1u;
+  }
+  ctl->ring[0][0][0] = r0;
+  zero_ring(&ctl->ring[0][0][1]);
+  zero_ring(&ctl->ring[0][0][2]);
+  VisitState n = S;
+  n.phase = kPhaseLevel;
+  n.lvl = 0;
+  n.steps = n.td = n.bu = n.maxf = 0;
+  n.launches = 1;
+  n.bu_mode = 0;
+  n.seen_edges = (uint64_t)d;
+  n.visited_v = 1;
+  n.visited_e = d;
+  n.expv = n.expe = 0;
+  *nxt = n;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void phase_kernel(VisitCtl* ctl, int parity) {
+  __shared__ VisitCtl h;
   __shared__ int32_t s_stage[kWavesPerBlock * kStageCap];
   __shared__ int32_t s_hub[2][kWavesPerBlock * kHubStage];
   __shared__ BlockShared s_sh;
+  __shared__ OffT s_root[4];
+  static_assert(sizeof(VisitCtl) == kWave * sizeof(uint4), "one 16-byte load per lane");
+  if (threadIdx.x < kWave)
+    reinterpret_cast<uint4*>(&h)[threadIdx.x] = reinterpret_cast<const uint4*>(ctl)[threadIdx.x];
+  __syncthreads();
+  const VisitParams<OffT>& p = *reinterpret_cast<const VisitParams<OffT>*>(h.params);
+  const VisitState& S = h.st[parity];
+  VisitState* nxt = &ctl->st[parity ^ 1];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (S.phase == kPhaseIdle) {
+    if (lead) nxt->phase = kPhaseIdle;
+    return;
+  }
+  const bool st = h.mode != kModeBfs;
+  int32_t* const parent = h.record_parents ? p.parent : nullptr;
+  if (S.phase == kPhaseReset) {
+    bfs_reset(p, S, nxt, parent);
+    return;
+  }
 
-  Decision d = ctl->dec;
-  if (phase == kPhaseReset) {
-    const int32_t src = ctl->src, dst = st ? ctl->dst : -1;
-    if (st || !p.lazy_state) {
-      for (int64_t v = gtid; v < p.V; v += nthr) {
-        int32_t s = -1;
-        if (v == src)
-          s = 0;
-        else if (v == dst)
-          s = kColorBit;
-        p.state[v] = s;
-        if (parent) parent[v] = s != -1 ? (int32_t)v : -1;
+  // ---- level phase: every block takes the same decisions from S ----
+  // (Scalars only below: a local copy of the state, or arrays indexed by the
+  // colour, would be put in scratch memory.)
+  int q = S.qi, L = S.lvl;
+  int qp = q & 1;
+  int32_t src = S.src, dst = S.dst;
+  uint32_t epoch = S.epoch;
+  bool start = st && L == 0;  // s–t: this launch expands the roots of query q
+  const int ncol = st ? 2 : 1;
+  Ring cur[2];
+  cur[0] = h.ring[qp][0][L % 3];
+  cur[1] = h.ring[qp][1][L % 3];
+  const uint32_t nf0 = ring_size(cur[0]), nf1 = st ? ring_size(cur[1]) : 0u;
+  const uint64_t mf01 = cur[0].mf + (st ? cur[1].mf : 0ull);
+  // the level-L frontier (built by the previous launch) counted as labelled
+  const bool counted = L > 0;
+  const uint64_t seen = S.seen_edges + (counted ? mf01 : 0ull);
+  if (!start) {
+    // see whether the query is finished
+    const bool empty = counted && (nf0 == 0 || (st && nf1 == 0));
+    const unsigned long long best = h.best[qp];
+    const bool runaway = S.steps > p.V + 2;
+    if ((st && best != ~0ull) || empty || h.err || runaway) {
+      if (lead) {
+        const bool met = st && best != ~0ull;
+        VisitOut o;
+        o.status = (h.err || runaway) ? FC_CAPACITY_STATUS : 0;
+        o.levels = S.steps;
+        o.depth = met ? S.steps : 0;
+        o.dist = met ? (int32_t)(best >> 32) : -1;
+        o.meet = met ? (int32_t)(uint32_t)best : -1;
+        o.topdown = S.td;
+        o.bottomup = S.bu;
+        o.max_frontier = S.maxf;
+        o.visited_vertices = S.visited_v + (counted ? (int64_t)nf0 + (int64_t)nf1 : 0);
+        o.visited_edges = S.visited_e + (counted ? (int64_t)mf01 : 0);
+        o.expanded_vertices = S.expv;
+        o.expanded_edges = S.expe;
+        o.err = (int32_t)h.err;
+        // a BFS spends one more launch noticing its empty frontier; an s–t
+        // query notices its end in the launch that starts the next query
+        o.launches = S.launches + (st ? 0 : 1);
+        p.out[q] = o;
       }
-    } else if (gtid == 0) {
-      // BFS on a renumbered graph: only reached vertices' state and parent
-      // are written (the visited bitmap marks them), so only the root's here
-      p.state[src] = 0;
-      if (parent) parent[src] = src;
+      if (!st || q + 1 >= h.nq) {
+        if (lead) nxt->phase = kPhaseIdle;
+        return;
+      }
+      // the next s–t query starts in this launch
+      ++q;
+      qp = q & 1;
+      src = S.nsrc;
+      dst = S.ndst;
+      epoch = S.epoch + 1;
+      L = 0;
+      start = true;
     }
-    for (int64_t w = bitmaps ? gtid : (int64_t)nwords; w < nwords; w += nthr) {
-      const uint32_t bs = ((uint32_t)src >> 5) == (uint32_t)w ? 1u << (src & 31) : 0u;
-      const uint32_t bt = (dst >= 0 && ((uint32_t)dst >> 5) == (uint32_t)w) ? 1u << (dst & 31) : 0u;
-      p.visited[w] = bs | bt;
-      p.fbm[0][0][w] = bs;
-      p.fbm[0][1][w] = 0u;
-      p.fbm[0][2][w] = 0u;
-      if (st) {  // the second colour's frontier bitmaps serve s–t modes only
-        p.fbm[1][0][w] = bt;
-        p.fbm[1][1][w] = 0u;
-        p.fbm[1][2][w] = 0u;
+  }
+
+  const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t nthr = (int64_t)gridDim.x * kBlock;
+  const int gwave = (int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  const int nwaves = (int)(gridDim.x * kWavesPerBlock);
+  int strat = kTopDown;
+  int vw0 = 4, vw1 = 4;
+  if (start) {
+    if (threadIdx.x < 4) s_root[threadIdx.x] = p.off[(threadIdx.x < 2 ? src : dst) + (threadIdx.x & 1)];
+    __syncthreads();
+    const uint64_t ds = (uint64_t)(s_root[1] - s_root[0]), dt = (uint64_t)(s_root[3] - s_root[2]);
+    if (lead) {
+      // the new query's machine state after this launch (its level 0)
+      nxt->phase = kPhaseLevel;
+      nxt->qi = q;
+      nxt->src = src;
+      nxt->dst = dst;
+      nxt->nsrc = q + 1 < h.nq ? p.qs[q + 1] : -1;
+      nxt->ndst = q + 1 < h.nq ? p.qt[q + 1] : -1;
+      nxt->lvl = 1;
+      nxt->epoch = epoch;
+      nxt->steps = 1;
+      nxt->td = 2;
+      nxt->bu = 0;
+      nxt->maxf = 1;
+      nxt->launches = 1;
+      nxt->bu_mode = 0;
+      nxt->seen_edges = ds + dt;
+      nxt->visited_v = 2;
+      nxt->visited_e = (int64_t)(ds + dt);
+      nxt->expv = 2;
+      nxt->expe = (int64_t)(ds + dt);
+      p.sst[src] = st_word(epoch, 0, 0);
+      p.sst[dst] = st_word(epoch, 1, 0);
+      if (parent) {
+        parent[src] = src;
+        parent[dst] = dst;
       }
+      // the next query's counters (this query's level-1 ring was cleared when
+      // the previous query started), and this query's level-2 ring
+      for (int c = 0; c < 2; ++c) {
+        for (int k = 0; k < 3; ++k) zero_ring(&ctl->ring[qp ^ 1][c][k]);
+        zero_ring(&ctl->ring[qp][c][2]);
+      }
+      ctl->best[qp ^ 1] = ~0ull;
     }
   } else {
-    const int lvl[2] = {ctl->lvl[0], ctl->lvl[1]};
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      if (!d.ex[c] || !bitmaps) continue;
-      uint32_t* far = p.fbm[c][(lvl[c] + 2) % 3];
-      for (int64_t w = gtid; w < nwords; w += nthr) far[w] = 0u;
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      if (!d.ex[c]) continue;
-      const Ring cur = ctl->ring[c][lvl[c] % 3];
-      const int cur_par = lvl[c] & 1;
-      Appender<OffT> a;
-      a.p = &p;
-      a.c = c;
-      a.nxt_par = cur_par ^ 1;
-      a.nxt = &ctl->ring[c][(lvl[c] + 1) % 3];
-      a.nfbm = p.fbm[c][(lvl[c] + 1) % 3];
-      a.bitmaps = bitmaps;
-      a.buf = s_stage + (threadIdx.x >> 6) * kStageCap;
-      a.cnt = 0;
-      a.mf = 0;
-      a.best = ~0ull;
-      a.h.u = &s_hub[0][(threadIdx.x >> 6) * kHubStage];
-      a.h.d = &s_hub[1][(threadIdx.x >> 6) * kHubStage];
-      a.h.cnt = 0;
-      if (d.strat[c] == kTopDown) {
-        td_dispatch(a, parent, st, d.vw[c], lvl[c], cur_par, cur.n, gwave, nwaves);
-        td_hubs(a, parent, st, lvl[c], cur_par, cur.hub, gwave, nwaves);
-      } else {
-        const uint32_t* ofbm = st ? p.fbm[1 - c][lvl[1 - c] % 3] : nullptr;
-        bu_step(a, parent, st, lvl[c], p.fbm[c][lvl[c] % 3], ofbm, gwave, nwaves);
+    const double mu = (double)p.E - (double)seen;
+    vw0 = h.vw_fixed > 0 ? h.vw_fixed : degree_width(cur[0].n, cur[0].mf - (cur[0].hub & kHubEdgeMask));
+    if (st) vw1 = h.vw_fixed > 0 ? h.vw_fixed : degree_width(cur[1].n, cur[1].mf - (cur[1].hub & kHubEdgeMask));
+    int bu = S.bu_mode;
+    if (!st) {
+      if (!bu) {
+        if ((double)cur[0].mf > mu / kAlpha) bu = 1;
+      } else if ((double)nf0 * kBeta < (double)p.V) {
+        bu = 0;
       }
-      block_flush(a, s_sh);
+      strat = bu ? kBottomUp : kTopDown;
+    }
+    if (lead) {
+      nxt->phase = kPhaseLevel;
+      nxt->qi = q;
+      nxt->src = src;
+      nxt->dst = dst;
+      nxt->nsrc = S.nsrc;
+      nxt->ndst = S.ndst;
+      nxt->lvl = L + 1;
+      nxt->epoch = epoch;
+      nxt->steps = S.steps + 1;
+      nxt->td = S.td + (strat == kTopDown ? ncol : 0);
+      nxt->bu = S.bu + (strat == kTopDown ? 0 : ncol);
+      const int32_t mx = (int32_t)(nf0 > nf1 ? nf0 : nf1);
+      nxt->maxf = mx > S.maxf ? mx : S.maxf;
+      nxt->launches = S.launches + 1;
+      nxt->bu_mode = bu;
+      nxt->seen_edges = seen;
+      nxt->visited_v = S.visited_v + (counted ? (int64_t)nf0 + (int64_t)nf1 : 0);
+      nxt->visited_e = S.visited_e + (counted ? (int64_t)mf01 : 0);
+      nxt->expv = S.expv + (int64_t)nf0 + (int64_t)nf1;
+      nxt->expe = S.expe + (int64_t)mf01;
+      zero_ring(&ctl->ring[qp][0][(L + 2) % 3]);
+      if (st) zero_ring(&ctl->ring[qp][1][(L + 2) % 3]);
+    }
+    if (!st) {
+      // the BFS frontier bitmap two levels ahead is free: clear it for the
+      // next level's appends
+      uint32_t* far = p.fbm[0][(L + 2) % 3];
+      const uint32_t nwords = (((uint32_t)p.V + 63u) >> 6) * 2u;
+      for (int64_t w = gtid; w < nwords; w += nthr) far[w] = 0u;
     }
   }
 
-  // Arrival: every wave drains its own memory operations (including
-  // non-returning atomics), then thread 0 takes a ticket; the block with the
-  // last ticket finalises. The finaliser reads only counters that other
-  // blocks updated with device-scope atomics (ring, best, err), all
-  // performed before their tickets, so no cache writeback is needed here;
-  // the plain stores of this launch (queues, states, bitmaps) are consumed
-  // by the next launch, after the kernel boundary.
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // Two-level ticket: the blocks of group b % kTicketGroups count on their
-    // own line, and the last of each group counts on the launch ticket, so
-    // no single address takes an atomic from every block.
-    const uint32_t k = blockIdx.x % kTicketGroups;
-    const uint32_t gsize = (nb - k + kTicketGroups - 1) / kTicketGroups;
-    uint32_t* sub = &ctl->gticket[k * 32];
-    const uint32_t t1 = __hip_atomic_fetch_add(sub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool last = false;
-    if (t1 == gsize - 1) {
-      __hip_atomic_store(sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t groups = nb < (uint32_t)kTicketGroups ? nb : (uint32_t)kTicketGroups;
-      const uint32_t t = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = t == groups - 1;
-    }
-    s_last = last;
+  // The waves are split between the colours in proportion to their frontier
+  // edges, so that both colours' chains of dependent loads run at the same
+  // time; each wave expands one colour and stages into that colour's lists.
+  const int cur_par = L & 1;
+  int w0 = nwaves;
+  if (ncol == 2) {
+    const double e0 = start ? (double)(s_root[1] - s_root[0]) : (double)cur[0].mf;
+    const double e1 = start ? (double)(s_root[3] - s_root[2]) : (double)cur[1].mf;
+    const int lo = e0 > 0 ? 1 : 0, hi = e1 > 0 ? nwaves - 1 : nwaves;
+    const double f = e0 + e1 > 0 ? e0 / (e0 + e1) : 0.5;
+    w0 = (int)((double)nwaves * f + 0.5);
+    w0 = w0 < lo ? lo : (w0 > hi ? hi : w0);
   }
-  __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
-  if (phase == kPhaseReset)
-    finalize_reset(p);
-  else
-    finalize_level(p, d);
-  __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int cw = gwave < w0 ? 0 : 1;  // this wave's colour
+  const int lw = cw ? gwave - w0 : gwave, nlw = cw ? nwaves - w0 : w0;
+  Appender<OffT> a;
+  a.p = &p;
+  a.c = cw;
+  a.nxt_par = cur_par ^ 1;
+  a.nxt = &ctl->ring[qp][cw][(L + 1) % 3];
+  a.nfbm = p.fbm[cw][(L + 1) % 3];
+  a.best_out = &ctl->best[qp];
+  a.bitmaps = !st;
+  a.epoch = epoch;
+  a.root_own = cw ? dst : src;
+  a.root_other = cw ? src : dst;
+  a.buf = s_stage + (threadIdx.x >> 6) * kStageCap;
+  a.cnt = 0;
+  a.mf = 0;
+  a.best = ~0ull;
+  a.h.u = &s_hub[0][(threadIdx.x >> 6) * kHubStage];
+  a.h.d = &s_hub[1][(threadIdx.x >> 6) * kHubStage];
+  a.h.cnt = 0;
+  if (start) {
+    td_root(a, parent, cw ? dst : src, s_root[2 * cw], s_root[2 * cw + 1], lw, nlw);
+  } else if (strat == kTopDown) {
+    // selects, not indexing: a local array indexed by a runtime value
+    // would live in scratch memory
+    const int vwc = cw ? vw1 : vw0;
+    const uint32_t nc = cw ? cur[1].n : cur[0].n;
+    const uint64_t hc = cw ? cur[1].hub : cur[0].hub;
+    td_dispatch(a, parent, st, vwc, L, cur_par, nc, lw, nlw);
+    td_hubs(a, parent, st, L, cur_par, hc, lw, nlw);
+  } else {
+    bu_step(a, parent, L, p.fbm[cw][L % 3], lw, nlw);
+  }
+  block_flush(a, ncol, s_sh, &ctl->ring[qp][0][(L + 1) % 3], &ctl->ring[qp][1][(L + 1) % 3]);
 }
 
 template <typename OffT>
-hipError_t launch_phase(const VisitParams<OffT>& p, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(phase_kernel<OffT>, dim3(blocks), dim3(kBlock), 0, s, p);
+hipError_t launch_phase(VisitCtl* ctl, int parity, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(phase_kernel<OffT>, dim3(blocks), dim3(kBlock), 0, s, ctl, parity);
   return hipGetLastError();
 }
 
@@ -769,8 +860,8 @@ int phase_occupancy() {
   return nb;
 }
 
-template hipError_t launch_phase<int32_t>(const VisitParams<int32_t>&, int, hipStream_t);
-template hipError_t launch_phase<int64_t>(const VisitParams<int64_t>&, int, hipStream_t);
+template hipError_t launch_phase<int32_t>(VisitCtl*, int, int, hipStream_t);
+template hipError_t launch_phase<int64_t>(VisitCtl*, int, int, hipStream_t);
 template int phase_occupancy<int32_t>();
 template int phase_occupancy<int64_t>();
 
