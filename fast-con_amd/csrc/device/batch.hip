@@ -40,7 +40,11 @@ namespace fcd {
 using fc::fail;
 
 constexpr uint32_t kBStage = 512;  // staged entries per wave and list
-constexpr float kDefaultLearnMargin = 0.1f;
+// Margin of the learned policy when the caller gives none: the value model
+// selection chose on the validation graphs for the shipped parameters
+// (fastcon/sched_report.json), so C callers, the CLIs and the Python binding
+// all get the same behaviour.
+constexpr float kDefaultLearnMargin = 1.0f;
 constexpr int kMaxLanes = 4;          // concurrent wave machines per batch
 constexpr int kDefaultLanes = 3;
 // A filter launch pays when the expected number of dropped list entries or
@@ -311,17 +315,27 @@ __device__ __forceinline__ void expand_normal(const BatchParams<OffT>& p, int x,
 }
 
 // Hub frontier vertices are expanded edge-parallel: the concatenated
-// adjacency of the hub list is cut into chunks of `chunk` edges, one chunk per
-// wave at a time, four edges per lane and pass.
+// adjacency of the hub list is cut into chunks of `chunk` edges (the tile),
+// one chunk per wave at a time, four edges per lane and pass. When the hub
+// edges are too few for every wave to get a whole tile, the chunk is halved
+// (down to 64 edges, staying a multiple of 64) so that the work spreads over
+// more waves: a wave's chain of dependent loads per pass, not the edge
+// count, then sets the phase's time (the roots' expansion at the start of a
+// wave has a few thousand hub edges).
 template <typename OffT>
 __device__ __forceinline__ void expand_hubs(const BatchParams<OffT>& p, int x, int par, unsigned long long hubpk,
-                                            unsigned long long act, int chunk, int gwave, int nwaves,
+                                            unsigned long long act, int tile, int gwave, int nwaves,
                                             unsigned long long& a_cnt, unsigned long long& e_cnt) {
   constexpr int R = 4;
   uint32_t nh = (uint32_t)(hubpk >> kHubEdgeBits);
   if (nh > p.hcap) nh = p.hcap;
   const uint64_t H = hubpk & kHubEdgeMask;
   if (nh == 0 || H == 0) return;
+  int chunk = tile;
+  while (chunk > kWave && (uint64_t)chunk * (uint64_t)nwaves > H) {
+    const int half = (chunk >> 1) & ~(kWave - 1);
+    chunk = half > kWave ? half : kWave;
+  }
   const int lane = lane_id();
   const int32_t* hl = p.hl[x][par];
   const int64_t* hb = p.hb[x][par];
@@ -914,7 +928,11 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
         ctl->cur[x] = 0;
       }
       ctl->act = 0;
-      ctl->wave++;
+      ctl->waves_done++;
+      // the next wave: claimed from the counter the concurrent machines
+      // share (so that a machine whose waves ran fast takes more of them),
+      // or this machine's next one
+      ctl->wave = p.wave_ticket ? (int32_t)atomicAdd(p.wave_ticket, 1u) : ctl->wave + 1;
       ctl->t_wave_end = (int64_t)wall_clock64();
       ctl->phase = ctl->wave < ctl->nwaves && ctl->wave < ctl->stop_wave ? kBInit : kBIdle;
       return kFinDone;
@@ -1249,8 +1267,16 @@ struct BatchWorkspace {
   int32_t* qs = nullptr;
   int32_t* qt = nullptr;
   int32_t* dist = nullptr;
+  uint32_t* wticket = nullptr;  // next unclaimed wave (lane 0's is shared by all lanes of a run)
   int32_t qcap = 0;
   int blocks = 0;
+  // Query list, results and wave counter of the current run: lane 0's for
+  // every lane. A captured graph holds the pointers it was captured with.
+  const int32_t* run_qs = nullptr;
+  const int32_t* run_qt = nullptr;
+  int32_t* run_dist = nullptr;
+  uint32_t* run_ticket = nullptr;
+  const void* cap_key[4] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_force = -2;
@@ -1312,16 +1338,13 @@ static fc_status batch_alloc_lane(fc_graph* g, BatchWorkspace** out, bool own_st
   o += al(4 * ((V + 63) / 32));
   const size_t o_ctl = o;
   o += al(sizeof(BatchCtl));
+  const size_t o_wt = o;
+  o += al(sizeof(uint32_t));
   b->bytes = o;
   hipError_t e = hipMalloc((void**)&b->mem, b->bytes);
   if (e != hipSuccess) {
     delete b;
     return hip_fail(e, "batch workspace allocation");
-  }
-  e = hipMemset(b->mem, 0, b->bytes);
-  if (e != hipSuccess) {
-    batch_free(b);
-    return hip_fail(e, "batch workspace clear");
   }
   b->rec = (unsigned long long*)(b->mem + o_rec);
   for (int x = 0; x < 2; ++x) {
@@ -1336,6 +1359,7 @@ static fc_status batch_alloc_lane(fc_graph* g, BatchWorkspace** out, bool own_st
   b->red = (unsigned long long*)(b->mem + o_red);
   b->db = (uint32_t*)(b->mem + o_db);
   b->ctl = (BatchCtl*)(b->mem + o_ctl);
+  b->wticket = (uint32_t*)(b->mem + o_wt);
   int occ = g->off64 ? batch_occupancy<int64_t>() : batch_occupancy<int32_t>();
   if (occ < 1) {
     batch_free(b);
@@ -1360,6 +1384,16 @@ static fc_status batch_alloc_lane(fc_graph* g, BatchWorkspace** out, bool own_st
     b->own_stream = true;
   } else {
     b->stream = g->stream;
+  }
+  // Cleared on the lane's own stream and finished before the first use: the
+  // lanes' streams do not wait for the legacy null stream, and a plain
+  // hipMemset there may still be running when the lane writes its control
+  // block (the last bytes of the allocation).
+  e = hipMemsetAsync(b->mem, 0, b->bytes, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    batch_free(b);
+    return hip_fail(e, "batch workspace clear");
   }
   if (hipEventCreate(&b->ev_end) != hipSuccess) {
     b->ev_end = nullptr;
@@ -1435,9 +1469,10 @@ static BatchParams<OffT> batch_params(fc_graph* g, BatchWorkspace* b) {
   p.red = b->red;
   p.db = b->db;
   p.ctl = b->ctl;
-  p.qs = b->qs;
-  p.qt = b->qt;
-  p.dist = b->dist;
+  p.qs = b->run_qs;
+  p.qt = b->run_qt;
+  p.dist = b->run_dist;
+  p.wave_ticket = b->run_ticket;
   p.perm = g->relabeled ? g->d_perm : nullptr;
   p.v_active = (int32_t)g->v_active;
   p.mlp = g->d_mlp + kMlpParams;  // the copy in mlp_t_index order
@@ -1449,8 +1484,22 @@ static hipError_t batch_launch_raw(fc_graph* g, BatchWorkspace* b) {
   return launch_batch_phase<int32_t>(batch_params<int32_t>(g, b), b->blocks, b->stream);
 }
 
+// Point a lane at the run's query list, results and wave counter.
+static void batch_bind(BatchWorkspace* b, const int32_t* qs, const int32_t* qt, int32_t* dist, uint32_t* ticket) {
+  b->run_qs = qs;
+  b->run_qt = qt;
+  b->run_dist = dist;
+  b->run_ticket = ticket;
+}
+
 static fc_status batch_graph(fc_graph* g, BatchWorkspace* b) {
-  if (b->gexec) return FC_OK;
+  const void* key[4] = {b->run_qs, b->run_qt, b->run_dist, b->run_ticket};
+  if (b->gexec && std::memcmp(key, b->cap_key, sizeof(key)) == 0) return FC_OK;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  std::memcpy(b->cap_key, key, sizeof(key));
   FC_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
   for (int i = 0; i < kGraphLaunches && le == hipSuccess; ++i) le = batch_launch_raw(g, b);
@@ -1465,11 +1514,11 @@ static fc_status batch_graph(fc_graph* g, BatchWorkspace* b) {
 
 // Prepare the control block for a batch of n queries (already on the device).
 fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int vw, int tile, int trace, int every,
-                      float margin) {
+                      float margin, int32_t first_wave) {
   BatchCtl init;
   std::memset(&init, 0, sizeof(init));
   init.phase = kBInit;
-  init.wave = 0;
+  init.wave = first_wave;
   init.nwaves = (n + 63) / 64;
   init.nq = n;
   init.policy = policy;
@@ -1501,7 +1550,8 @@ fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int
 // for to the point where every lane has finished.
 struct LaneRun {
   BatchWorkspace* b;
-  int64_t nw;
+  int64_t nw;        // waves this lane is expected to take
+  int64_t nw_total;  // waves of the batch
   int64_t expected;
   int64_t launched;
   int round;
@@ -1550,10 +1600,11 @@ static fc_status batch_drive(fc_graph* g, LaneRun* lr, int nl, bool use_graph, f
         L.done = true;
         continue;
       }
-      if (L.launched > L.nw * (4 * g->V + 16) + 64)
+      if (L.launched > L.nw_total * (4 * g->V + 16) + 64)
         return fail(FC_ERR_CAPACITY, "batch phase machine did not finish after %lld launches", (long long)L.launched);
-      const int64_t remaining = std::max<int64_t>(L.nw - L.b->h_phase[1], 1);
-      L.expected = L.launched + (int64_t)std::ceil((double)remaining * (L.b->est_launches + 4.0) * (1.0 + L.round)) + 2;
+      // Waves are claimed dynamically, so a lane's remaining work is not
+      // known: budget one more wave, then more each round.
+      L.expected = L.launched + (int64_t)std::ceil((L.b->est_launches + 4.0) * (1.0 + L.round)) + 2;
       L.round++;
     }
   }
@@ -1567,50 +1618,59 @@ static fc_status batch_drive(fc_graph* g, LaneRun* lr, int nl, bool use_graph, f
   return FC_OK;
 }
 
-// Answer n queries: the waves are split between `lanes` concurrent wave
-// machines (contiguous ranges of whole waves); distances land in lane 0's
-// pinned h_dist[0..n).
+// Answer n queries with `lanes` concurrent wave machines. Every lane reads
+// lane 0's query list and writes lane 0's results; lane k starts with wave
+// k and each lane, when it finishes a wave, claims the next unclaimed one
+// from a counter in lane 0's workspace, so lanes that meet cheap waves take
+// more of them. Distances land in lane 0's pinned h_dist[0..n).
 static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy, int vw, int tile,
                            bool use_graph, float* ms, BatchCtl* ctl_out, int64_t* launches = nullptr,
                            float margin = -1.f, int lanes = 1) {
   const int64_t nw = (n + 63) / 64;
   const int nl = (int)std::max<int64_t>(1, std::min<int64_t>(lanes, nw));
   LaneRun lr[kMaxLanes];
-  int32_t first[kMaxLanes + 1];
   fc_status st;
-  for (int k = 0; k <= nl; ++k) first[k] = (int32_t)std::min<int64_t>((int64_t)n, 64 * ((nw * k) / nl));
+  BatchWorkspace* b0 = nullptr;
+  if ((st = batch_lane(g, 0, &b0)) != FC_OK) return st;
+  if ((st = batch_queries(g, b0, n)) != FC_OK) return st;
+  FC_HIP(hipMemcpyAsync(b0->qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b0->stream));
+  FC_HIP(hipMemcpyAsync(b0->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b0->stream));
+  const uint32_t first_unclaimed = (uint32_t)nl;
+  if (nl > 1)
+    FC_HIP(hipMemcpyAsync(b0->wticket, &first_unclaimed, sizeof(uint32_t), hipMemcpyHostToDevice, b0->stream));
   for (int k = 0; k < nl; ++k) {
     BatchWorkspace* b = nullptr;
     if ((st = batch_lane(g, k, &b)) != FC_OK) return st;
-    const int32_t m = first[k + 1] - first[k];
-    if ((st = batch_queries(g, b, k == 0 ? n : m)) != FC_OK) return st;
+    batch_bind(b, b0->qs, b0->qt, b0->dist, nl > 1 ? b0->wticket : nullptr);
     if (use_graph && (st = batch_graph(g, b)) != FC_OK) return st;
-    FC_HIP(hipMemcpyAsync(b->qs, s + first[k], sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, b->stream));
-    FC_HIP(hipMemcpyAsync(b->qt, t + first[k], sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, b->stream));
-    if ((st = batch_begin(g, b, m, policy, vw, tile, 0, 1, margin)) != FC_OK) return st;
+    if ((st = batch_begin(g, b, n, policy, vw, tile, 0, 1, margin, k)) != FC_OK) return st;
     lr[k].b = b;
-    lr[k].nw = (m + 63) / 64;
+    lr[k].nw = (nw + nl - 1 - k) / nl;
+    lr[k].nw_total = nw;
   }
+  // the copies above read pageable host memory and the lanes read lane 0's
+  // arrays: every stream is idle before the first launch
   for (int k = 0; k < nl; ++k) FC_HIP(hipStreamSynchronize(lr[k].b->stream));
   if ((st = batch_drive(g, lr, nl, use_graph, ms, launches)) != FC_OK) return st;
-  BatchWorkspace* b0 = lr[0].b;
   std::vector<BatchCtl> cs(nl);
-  for (int k = 0; k < nl; ++k) {
-    const int32_t m = first[k + 1] - first[k];
-    FC_HIP(hipMemcpyAsync(b0->h_dist + first[k], lr[k].b->dist, sizeof(int32_t) * (size_t)m, hipMemcpyDeviceToHost,
-                          lr[k].b->stream));
+  FC_HIP(hipMemcpyAsync(b0->h_dist, b0->dist, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, b0->stream));
+  for (int k = 0; k < nl; ++k)
     FC_HIP(hipMemcpyAsync(&cs[k], lr[k].b->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, lr[k].b->stream));
-  }
   for (int k = 0; k < nl; ++k) FC_HIP(hipStreamSynchronize(lr[k].b->stream));
   uint32_t err = 0;
   for (int k = 0; k < nl; ++k) err |= cs[k].err;
   if (err) return fail(FC_ERR_CAPACITY, "batch kernel: list guard tripped (err=%u)", err);
+  int64_t done = 0;
+  for (int k = 0; k < nl; ++k) done += cs[k].waves_done;
+  if (done != nw)
+    return fail(FC_ERR_HIP, "batch: %lld of %lld waves finished (lane 0: phase %d, wave %d of %d, %lld working launches)",
+                (long long)done, (long long)nw, cs[0].phase, cs[0].wave, cs[0].nwaves, (long long)cs[0].phases);
   // Running estimate of the working launches (finalised phases) per wave.
   // It follows the phases, not the launches issued: those are rounded up to
   // whole captured graphs, and an estimate fed by them would only grow.
   for (int k = 0; k < nl; ++k)
-    if (lr[k].nw > 0)
-      lr[k].b->est_launches = 0.5 * lr[k].b->est_launches + 0.5 * ((double)cs[k].phases / (double)lr[k].nw);
+    if (cs[k].waves_done > 0)
+      lr[k].b->est_launches = 0.5 * lr[k].b->est_launches + 0.5 * ((double)cs[k].phases / (double)cs[k].waves_done);
   if (ctl_out) {
     *ctl_out = cs[0];
     for (int k = 1; k < nl; ++k) {
@@ -1797,9 +1857,10 @@ extern "C" fc_status fc_sched_trace(fc_graph* g, const int32_t* s, const int32_t
   if (khz <= 0) return fc::fail(FC_ERR_HIP, "fc_sched_trace: device clock rate unknown");
   FC_HIP(hipMemcpyAsync(b->qs, s, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
   FC_HIP(hipMemcpyAsync(b->qt, t, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+  fcd::batch_bind(b, b->qs, b->qt, b->dist, nullptr);
   const int vw = b->tuned_vw >= 0 ? b->tuned_vw : 0;
   const int tile = b->tuned_tile > 0 ? b->tuned_tile : 256;
-  if ((st = fcd::batch_begin(g, b, n, policy, vw, tile, cap > 0 ? 1 : 0, every, 0.f)) != FC_OK) return st;
+  if ((st = fcd::batch_begin(g, b, n, policy, vw, tile, cap > 0 ? 1 : 0, every, 0.f, 0)) != FC_OK) return st;
   const int64_t cap_launches = ((int64_t)n / 64 + 1) * (4 * g->V + 16) + 64;
   std::vector<BatchCtl> cv(2);
   BatchCtl& c = cv[0];

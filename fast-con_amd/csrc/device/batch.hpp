@@ -69,7 +69,7 @@ struct BatchCtl {
   float f_branch;          // feature 4: log2(1 + sum d^2 / sum d) / 16 of the graph
   int64_t vis_pairs[2];    // (vertex, query) pairs visited per side in this wave
   int32_t live0;           // live queries at the start of the wave
-  int32_t pad1;
+  int32_t waves_done;      // waves this machine has finished
   int64_t t_decide;        // device clock at the last kBDecide
   int64_t t_wave_end;      // device clock at the last wave cleanup
   unsigned long long lact[2];  // live queries when side x's current lists were built
@@ -121,6 +121,8 @@ struct BatchParams {
   const int32_t* qt;
   const int32_t* perm;         // caller id -> device id (null: the same)
   int32_t* dist;               // results (device), nq entries
+  uint32_t* wave_ticket;       // next unclaimed wave, shared by concurrent machines (null: this
+                               // machine takes waves wave, wave + 1, ... up to stop_wave)
   const float* mlp;
   int32_t v_active;            // vertices [0, v_active) have degree > 0 (the rest are isolated)
   int32_t pad;
