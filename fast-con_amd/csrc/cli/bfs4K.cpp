@@ -3,8 +3,12 @@
 //   bfs4K <graph_path> [ -D | -U ] [ -n <number_of_tests> ]
 //
 // Arguments, errors and the summary lines follow BFS-4K/bfs4K.cpp:58-81 and
-// BFS-4K/src/device/BFS_WorkEfficient.cu:6-66. Differences, each because the
-// reference's behaviour is not reproducible or not defined by its output:
+// BFS-4K/src/device/BFS_WorkEfficient.cu:6-66. As in the reference, 100
+// sources are run by default (N_OF_TESTS, BFS-4K/config.h:7). Differences,
+// each because the reference's behaviour is not reproducible, not defined by
+// its output or not reachable:
+//   * -n <number_of_tests> is honoured (the reference parses it but always
+//     runs N_OF_TESTS, BFS-4K/bfs4K.cpp:54,74-76);
 //   * sources are drawn from a seeded generator (environment FASTCON_SEED,
 //     default 1) instead of the time-seeded Mersenne twister; as in the
 //     reference, one test uses source 0 and sources reaching fewer than
@@ -67,7 +71,7 @@ uint64_t splitmix(uint64_t& st) {
 int main(int argc, char* argv[]) {
   if (argc < 2) die(kSyntax);
   fc_direction dir = FC_DIR_FILE;
-  int nof_tests = 1;
+  int nof_tests = 100;
   for (int i = 2; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "-D")

@@ -227,7 +227,7 @@ fc_status fc_stcon_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t
  * device memory, used in the caller's numbering and adjacency order; s, t,
  * connected, dist and stats are host memory as in fc_stcon_many, with the
  * same verdicts and distances. The per-query workspace is one fixed device
- * array of about 23 MB that every GPU the library runs on holds for the life
+ * array of about 27 MB that every GPU the library runs on holds for the life
  * of the process. Limits: 1 <= V <= 2^20 and E <= 2^25 (FC_ERR_INVALID
  * beyond them). The call creates no stream, event or graph; it launches on
  * the null stream and returns when done. Every call on one GPU shares that
@@ -292,7 +292,9 @@ typedef struct fc_batch_opts {
   int32_t words;         /* 64-bit query words per vertex: 1 (64 queries per wave) or 0 (the
                             same); other values are FC_ERR_UNSUPPORTED */
   float   learn_margin;  /* learned policy: the network overrides the heuristic's action only
-                            when it predicts a log-cost lower by at least this; < 0 = 0.1 */
+                            when it predicts a log-cost lower by at least this; < 0 = 1.0, the
+                            margin chosen on validation graphs for the shipped parameters (at
+                            which the network follows the heuristic on held-out graphs) */
   int32_t lanes;         /* waves run concurrently, each on its own stream and workspace
                             (1..4); 0 = default (3) */
   int32_t reserved;

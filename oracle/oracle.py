@@ -15,13 +15,20 @@ What pins each oracle function (so that a plausible slip fails a check):
   canonical       set semantics vs a Python set-of-pairs model; the s16
                   directed-edge count 1,831,864 of the Graph500 graph
   stcon           symmetry d(s,t) = d(t,s); the s16 known answers
-  stcon_work      brute-force enumeration on small graphs
-  validate_bfs_tree  planted mistakes (wrong parent, wrong level, missing
-                  edge) are each rejected
-  count_visited_edges  equals E/2 + self-loops/2 on connected canonical graphs
+  stcon_work      brute-force enumeration on small graphs; the GPU plain
+                  visit's expanded counters equal it on connected pairs
+  validate_bfs_tree  planted mistakes (root, wrong parent, wrong level,
+                  non-edge, an unreached vertex with a parent, an edge
+                  across the tree's boundary or across two levels) are each
+                  rejected with their rule's code
+  count_visited_edges  a hand-worked graph with self-loops and an unreached
+                  component (a < for <= or a dropped reached test fails it),
+                  and E/2 on a connected loop-free grid
   get_statistics  numpy mean/std(ddof=1)/min/max/quantile rules
   find_stcon_roots  the 100 (s,t) pairs of the s16 known-answer file
-  batch_wave      distances equal bfs-based distances per query
+  batch_wave      distances equal bfs-based distances per query; its A/E and
+                  iteration counters follow the closed forms of a path and
+                  a star
 """
 from __future__ import annotations
 

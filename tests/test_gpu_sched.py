@@ -131,11 +131,15 @@ def test_trace_keeps_answers_and_times_every_action(fc, orc):
             assert np.array_equal(g.stcon_batch(s, t, policy=pol).dist, exp)
 
 
-def test_trained_network_gains_over_plain_schedule(fc, orc):
-    """The shipped trained parameters on a graph and queries they were not
-    trained on: every answer equals the oracle's, and the learned schedule
-    beats the plain schedule (both frontiers top-down every step, the
-    reference's order of work)."""
+def test_trained_network_against_heuristic_and_plain(fc, orc):
+    """The shipped trained parameters at their validated margin, on a graph
+    and queries they were not trained on: every answer equals the oracle's;
+    the learned schedule beats the plain schedule (both frontiers top-down
+    every step, the reference's order of work); and against the heuristic,
+    the method without the network, it gains nothing: at the shipped margin
+    the network takes the heuristic's decisions on held-out graphs and adds
+    only its inference, about 1-5% of a batch. The measured ratio is printed;
+    the test bounds the overhead at 10%."""
     hg = fc.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=77)
     r = hg.sample_roots(1024, 78)
     s, t = r[:512], r[512:]
@@ -152,5 +156,7 @@ def test_trained_network_gains_over_plain_schedule(fc, orc):
                 assert np.array_equal(res.dist[:48], exp), name
                 times.append(res.stats["time_ms"])
             ms[name] = float(np.median(times))
-        say(f"rmat20 512 queries, ms per batch: {ms}; learned gains {ms['plain'] / ms['learned']:.2f}x over plain")
+        say(f"rmat20 512 queries, ms per batch: {ms}; learned vs plain {ms['plain'] / ms['learned']:.2f}x, "
+            f"learned vs heuristic {ms['heuristic'] / ms['learned']:.3f}x")
         assert ms["learned"] * 1.05 < ms["plain"]
+        assert ms["learned"] < 1.10 * ms["heuristic"]
