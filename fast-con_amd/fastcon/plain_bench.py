@@ -21,6 +21,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--graph", choices=["grid", "rmat"], default="grid")
     ap.add_argument("--pairs", type=int, default=32)
+    ap.add_argument("--no-batch", action="store_true", help="skip the comparison with the batched engine")
     args = ap.parse_args()
     if args.graph == "grid":
         hg = fc.gen_grid(4096, 4096, 0.02, 7)
@@ -38,6 +39,8 @@ def main():
         print(f"plain visit: {len(s)} pairs in {ms:.2f} ms = {len(s) / (ms / 1e3):.1f} q/s; "
               f"{st['launches'] / len(s):.1f} working launches and {st['levels'] / len(s):.1f} steps per query, "
               f"{ms * 1e3 / max(st['launches'], 1):.2f} us of device time per working launch", flush=True)
+        if args.no_batch:
+            return
         bres = g.stcon_batch(s, t)
         assert np.array_equal(res.dist, bres.dist), "plain and batched distances differ"
         print("distances agree with the batched engine", flush=True)
