@@ -653,6 +653,134 @@ __device__ __forceinline__ void filter_side(const BatchParams<OffT>& p, Appends&
   }
 }
 
+// Bridge step: before a side expands, test the edges between the two sides'
+// current frontiers directly. A query still live after the commit at levels
+// (i, j) has distance > i + j; an edge (a, b) with a in side s's frontier
+// and b in side t's, both carrying the query's bit, gives a path of length
+// i + 1 + j, so the distance is exactly i + j + 1 (the commit's "odd" meet
+// rule). The pairs come from the two frontier lists: tiles of 64 x 64 list
+// entries, one tile per wave at a time, with side t's entry broadcast
+// across the lanes; pairs with a common live bit are queued in the wave's
+// LDS and tested 64 at a time, each lane with a binary search of the
+// shorter id-sorted list. At R-MAT s22 about two thirds of the queries end
+// at distance 3, and the level-1 hubs that the next expansion would scan
+// (millions of edges) belong almost only to them: the queries that survive
+// a bridge carry about 5% of those edges. A pair left untested only leaves
+// its queries to the expansion, which finds the same distance, so each wave
+// tests at most kBridgeMaxTests pairs and the step's time stays bounded.
+constexpr uint32_t kBridgeQueue = 256;             // pairs queued per wave (16 bytes each, 4 KB)
+constexpr uint64_t kBridgeMaxPairs = 1ull << 26;   // list-entry pairs a bridge step may enumerate
+constexpr uint32_t kBridgeMaxTests = 1024;         // pairs a wave tests (16 rounds of 64 searches)
+constexpr uint64_t kBridgeMinEdges = 1ull << 20;   // live frontier edges of the cheaper side (default)
+static_assert(kBridgeQueue * 16 * kWavesPerBlock <= 2 * kWavesPerBlock * kBStage * sizeof(int32_t),
+              "bridge queues fit the appends' stage");
+
+// Whether x is in y's adjacency list, which is in ascending id order.
+template <typename OffT>
+__device__ __forceinline__ bool sorted_has(const BatchParams<OffT>& p, int32_t y, int32_t x) {
+  OffT lo = p.off[y];
+  const OffT end = p.off[y + 1];
+  OffT hi = end;
+  while (lo < hi) {
+    const OffT mid = lo + (hi - lo) / 2;
+    if (p.col[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < end && p.col[lo] == x;
+}
+
+// Test the queued pairs (wave-collective): OR the masks of adjacent pairs
+// into meets; each lane searches the list of the pair's lower-degree end.
+template <typename OffT>
+__device__ __forceinline__ void bridge_flush(const BatchParams<OffT>& p, const int32_t* qa, const int32_t* qb,
+                                             const unsigned long long* qm, uint32_t cnt,
+                                             unsigned long long& meets) {
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t base = 0; base < cnt; base += kWave) {
+    const uint32_t k = base + (uint32_t)lane_id();
+    if (k < cnt) {
+      const int32_t a = qa[k], b = qb[k];
+      const OffT da = p.off[a + 1] - p.off[a], db = p.off[b + 1] - p.off[b];
+      if (da <= db ? sorted_has(p, a, b) : sorted_has(p, b, a)) meets |= qm[k];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename OffT>
+__device__ __forceinline__ void bridge_pass(const BatchParams<OffT>& p, unsigned long long act, int gwave,
+                                            int nwaves, unsigned char* lds, unsigned long long& meets) {
+  const BatchCtl* ctl = p.ctl;
+  const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+  int32_t* qa = reinterpret_cast<int32_t*>(lds + (size_t)w * kBridgeQueue * 16);
+  int32_t* qb = qa + kBridgeQueue;
+  unsigned long long* qm = reinterpret_cast<unsigned long long*>(qb + kBridgeQueue);
+  const int c0 = ctl->cur[0], c1 = ctl->cur[1];
+  const uint32_t V = (uint32_t)p.V;
+  const uint32_t n0 = ctl->fl_n[0][c0] < V ? ctl->fl_n[0][c0] : V;
+  const uint32_t n1 = ctl->fl_n[1][c1] < V ? ctl->fl_n[1][c1] : V;
+  uint32_t h0 = (uint32_t)(ctl->hub[0][c0] >> kHubEdgeBits), h1 = (uint32_t)(ctl->hub[1][c1] >> kHubEdgeBits);
+  h0 = h0 < p.hcap ? h0 : p.hcap;
+  h1 = h1 < p.hcap ? h1 : p.hcap;
+  const uint32_t nS = n0 + h0, nT = n1 + h1;
+  const uint32_t ct = (nT + kWave - 1) / kWave;
+  const uint64_t tiles = (uint64_t)((nS + kWave - 1) / kWave) * ct;
+  uint32_t cnt = 0, tested = 0;
+  for (uint64_t tile = (uint64_t)gwave; tile < tiles && tested < kBridgeMaxTests; tile += (uint64_t)nwaves) {
+    const uint32_t i = (uint32_t)(tile / ct) * kWave + (uint32_t)lane;
+    const uint32_t j = (uint32_t)(tile % ct) * kWave + (uint32_t)lane;
+    int32_t a = -1, b = -1;
+    unsigned long long fa = 0, fb = 0;
+    if (i < nS) {
+      a = i < n0 ? p.fl[0][c0][i] : p.hl[0][c0][i - n0];
+      if ((uint32_t)a < V)
+        fa = p.F[0][c0][a] & act;
+      else
+        atomicOr(&p.ctl->err, 2u);
+    }
+    if (j < nT) {
+      b = j < n1 ? p.fl[1][c1][j] : p.hl[1][c1][j - n1];
+      if ((uint32_t)b < V)
+        fb = p.F[1][c1][b] & act;
+      else
+        atomicOr(&p.ctl->err, 2u);
+    }
+    uint64_t cols = ballot(fb != 0);
+    if (!ballot(fa != 0)) cols = 0;
+    while (cols) {
+      const int k = __builtin_ctzll(cols);
+      cols &= cols - 1;
+      const unsigned long long m = fa & (unsigned long long)shfl64((int64_t)fb, k);
+      const int32_t bk = __shfl(b, k);
+      const bool hit = m != 0 && a != bk;
+      const uint64_t hm = ballot(hit);
+      if (!hm) continue;
+      if (cnt + kWave > kBridgeQueue) {
+        bridge_flush(p, qa, qb, qm, cnt, meets);
+        tested += cnt;
+        cnt = 0;
+        if (tested >= kBridgeMaxTests) break;
+      }
+      if (hit) {
+        const uint32_t pos = cnt + lane_rank(hm);
+        qa[pos] = a;
+        qb[pos] = bk;
+        qm[pos] = m;
+      }
+      cnt += (uint32_t)__popcll(hm);
+    }
+  }
+  if (cnt) bridge_flush(p, qa, qb, qm, cnt, meets);
+}
+
+// Whether a bridge step runs before the pending step decision (lane 0 of the
+// finalising wave): not under the plain schedule, which is the reference's
+// order of work, and once per pair of levels.
+template <typename OffT>
+__device__ __forceinline__ bool bridge_due(const BatchParams<OffT>& p, const BatchCtl* ctl);
+
 // Features of the pending step for the learned scheduler (12 values in
 // about [0, 1]): per side the live frontier's size and degree sum relative
 // to the graph's (on a log scale), its mean degree relative to the graph's,
@@ -716,6 +844,21 @@ __device__ __forceinline__ uint64_t live_mf(const BatchCtl* ctl, int x) {
   const int live = __popcll(ctl->lact[x] & ctl->act);
   if (built == 0 || live == built) return mf;
   return (uint64_t)((double)mf * (double)live / (double)built);
+}
+
+template <typename OffT>
+__device__ __forceinline__ bool bridge_due(const BatchParams<OffT>& p, const BatchCtl* ctl) {
+  if (ctl->policy == 0 || !p.sorted_lists || !ctl->act) return false;
+  const int32_t key = ((ctl->lvl[0] + 1) << 16) | (ctl->lvl[1] + 1);
+  if (ctl->bridge_lv == key) return false;
+  uint64_t n[2];
+  for (int x = 0; x < 2; ++x) {
+    const int c = ctl->cur[x];
+    n[x] = (uint64_t)ctl->fl_n[x][c] + (ctl->hub[x][c] >> kHubEdgeBits);
+  }
+  if (!n[0] || !n[1] || n[0] * n[1] > kBridgeMaxPairs) return false;
+  const uint64_t m0 = live_mf(ctl, 0), m1 = live_mf(ctl, 1);
+  return (m0 < m1 ? m0 : m1) >= ctl->bridge_min_e;
 }
 
 // Heuristic (or plain) action for the pending step; also fills the feature
@@ -835,6 +978,7 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
       ctl->lvl[0] = ctl->lvl[1] = 0;
       ctl->live0 = __popcll(act);
       ctl->vis_pairs[0] = ctl->vis_pairs[1] = ctl->live0;
+      ctl->bridge_lv = 0;
       for (int x = 0; x < 2; ++x) {
         ctl->fl_n[x][0] = ln[x];
         ctl->hub[x][0] = lh[x];
@@ -887,6 +1031,16 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int ph
         ctl->phase = kBClearF;  // the consumed masks are cleared first
         return kFinDone;
       }
+      return batch_next(ctl, true);
+    }
+    case kBBridge: {
+      const unsigned long long act = ctl->act;
+      const unsigned long long hit = r.v[kROdd] & act;
+      const int base = ctl->lvl[0] + ctl->lvl[1];
+      for (unsigned long long m = hit; m; m &= m - 1) p.dist[wave0 + __builtin_ctzll(m)] = base + 1;
+      ctl->act = act & ~hit;
+      ctl->bridges++;
+      ctl->bridged += __popcll(hit);
       return batch_next(ctl, true);
     }
     case kBClearF: {
@@ -1081,6 +1235,11 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       block_reduce_slots(red, p.red, sh);
       break;
     }
+    case kBBridge: {
+      bridge_pass(p, act, gwave, nwaves, reinterpret_cast<unsigned char*>(&s_stage[0][0]), red[kROdd]);
+      block_reduce_slots(red, p.red, sh);
+      break;
+    }
     case kBClearF: {
       for (int x = 0; x < 2; ++x) {
         if (!(ctl->bu_sides & (1 << x))) continue;
@@ -1191,10 +1350,16 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
         ctl->pwork[ctl->phases] = (int64_t)(phase == kBCommit ? tot.v[kRTouched] : tot.v[kRActE]);
       }
       ctl->phases++;
-      if (phase == kBDecide)
+      if (phase == kBDecide) {
         need = ctl->pause_filter ? kFinDecideFilter : kFinDecide;
-      else
+      } else {
         need = batch_finalize(p, phase, d, tot, pre);
+        if (need != kFinDone && bridge_due(p, ctl)) {
+          ctl->bridge_lv = ((ctl->lvl[0] + 1) << 16) | (ctl->lvl[1] + 1);
+          ctl->phase = kBBridge;
+          need = kFinDone;
+        }
+      }
     }
   }
   need = __shfl(need, 0);
@@ -1475,6 +1640,7 @@ static BatchParams<OffT> batch_params(fc_graph* g, BatchWorkspace* b) {
   p.wave_ticket = b->run_ticket;
   p.perm = g->relabeled ? g->d_perm : nullptr;
   p.v_active = (int32_t)g->v_active;
+  p.sorted_lists = g->lists_sorted ? 1 : 0;
   p.mlp = g->d_mlp + kMlpParams;  // the copy in mlp_t_index order
   return p;
 }
@@ -1537,6 +1703,7 @@ fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int
   init.filt_min_v = env_u64("FASTCON_FILTER_MIN_VERTICES", kFilterMinVertices);
   init.filt_min_e = env_u64("FASTCON_FILTER_MIN_EDGES", kFilterMinEdges);
   init.both_max_e = env_u64("FASTCON_BOTH_MAX_EDGES", kBothMaxEdges);
+  init.bridge_min_e = env_u64("FASTCON_BRIDGE_MIN_EDGES", kBridgeMinEdges);
   FC_HIP(hipMemcpyAsync(b->ctl, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
   // the reduction slots are cleared by every finalisation; clear them here
   // too in case an earlier batch stopped on an error
@@ -1680,17 +1847,20 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
       ctl_out->touched_total += cs[k].touched_total;
       ctl_out->filters += cs[k].filters;
       ctl_out->phases += cs[k].phases;
+      ctl_out->bridges += cs[k].bridges;
+      ctl_out->bridged += cs[k].bridged;
     }
   }
   if (std::getenv("FASTCON_BATCH_LOG")) {
     // phase sequence of lane 0: I=init E=expand(s/t/b sides) C=commit U=bottom-up X=clear F=filter Z=cleanup
+    // D=traced decision B=bridge
     const BatchCtl& c = cs[0];
-    const char* nm = "?IECUXZF";
+    const char* nm = "?IECUXZFPDB?????";
     const int64_t np = std::min<int64_t>(c.phases, kPhaseLog);
     for (int64_t i = 0; i < np; ++i) {
-      const int ph = c.plog[i] >> 4, sd = c.plog[i] & 15;
+      const int ph = (uint8_t)c.plog[i] >> 4, sd = c.plog[i] & 15;
       const double us = i ? (double)(c.ptime[i] - c.ptime[i - 1]) / 100.0 : 0.0;  // 100 MHz clock
-      std::fprintf(stderr, "%c%s:%.0f", nm[ph & 7], sd == 1 ? "s" : sd == 2 ? "t" : sd == 3 ? "b" : "", us);
+      std::fprintf(stderr, "%c%s:%.0f", nm[ph & 15], sd == 1 ? "s" : sd == 2 ? "t" : sd == 3 ? "b" : "", us);
       if (c.pwork[i]) std::fprintf(stderr, "/%lldk", (long long)(c.pwork[i] / 1000));
       std::fputc(ph == kBCleanup ? '\n' : ' ', stderr);
     }
@@ -1818,6 +1988,8 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
     stats->filters = (int32_t)c.filters;
     stats->phases = (int32_t)c.phases;
     stats->launches = launches;
+    stats->bridges = c.bridges;
+    stats->bridged = (int32_t)c.bridged;
   }
   return FC_OK;
 }

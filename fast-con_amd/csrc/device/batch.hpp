@@ -18,7 +18,8 @@ enum BatchPhase : int32_t {
   kBCleanup = 6,   // zero every mask the wave touched, write results
   kBFilter = 7,    // drop resolved queries from the current frontier lists
   kBPause = 8,     // tracing: stopped at a step decision (launches are no-ops)
-  kBDecide = 9     // tracing: take the pending decision (forced or by policy)
+  kBDecide = 9,    // tracing: take the pending decision (forced or by policy)
+  kBBridge = 10    // test the edges between the two sides' current frontiers
 };
 
 enum BatchSides : int32_t { kSideS = 1, kSideT = 2, kSideBoth = 3 };
@@ -81,6 +82,10 @@ struct BatchCtl {
   unsigned long long filt_min_v;  // a filter runs when it drops at least this many list entries
   unsigned long long filt_min_e;  //   or this many frontier edges
   unsigned long long both_max_e;  // heuristic: both sides while their frontiers have at most this many edges
+  unsigned long long bridge_min_e;  // a bridge step runs before a step whose cheaper side has this many live edges
+  int32_t bridge_lv;       // (lvl[0] + 1) << 16 | (lvl[1] + 1) of this wave's last bridge step (0: none)
+  int32_t bridges;         // bridge steps (whole batch)
+  int64_t bridged;         // queries answered by bridge steps (whole batch)
   uint32_t ticket;
   uint32_t err;
   // statistics (whole batch)
@@ -125,7 +130,7 @@ struct BatchParams {
                                // machine takes waves wave, wave + 1, ... up to stop_wave)
   const float* mlp;
   int32_t v_active;            // vertices [0, v_active) have degree > 0 (the rest are isolated)
-  int32_t pad;
+  int32_t sorted_lists;        // every adjacency list is in ascending id order (bridge steps search them)
 };
 
 template <typename OffT>

@@ -302,6 +302,7 @@ static fc_status graph_create(const OffT* off, const int32_t* col, int64_t V, in
       if (noff[(size_t)v + 1] == noff[(size_t)v])
         return bail(fail(FC_ERR_INVALID, "internal: degree order broken at vertex %lld", (long long)v));
     g->v_active = va;
+    g->lists_sorted = !(flags & FC_GRAPH_KEEP_ORDER);
   } else {
     g->v_active = V;
     CK(hipMemcpyAsync(g->d_off, off, sizeof(OffT) * (size_t)(V + 1), hipMemcpyHostToDevice, g->stream));

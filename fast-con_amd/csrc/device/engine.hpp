@@ -63,6 +63,7 @@ struct fc_graph {
   // Degree relabelling (absent with FC_GRAPH_KEEP_IDS): the device graph
   // uses new ids; perm maps a caller id to its device id, iperm back.
   bool relabeled = false;
+  bool lists_sorted = false;  // device adjacency lists in ascending id order (renumbered, not FC_GRAPH_KEEP_ORDER)
   int64_t v_active = 0;  // vertices [0, v_active) have degree > 0: the degree-ordered
                          // numbering puts every isolated vertex after them; V otherwise
   int32_t* d_perm = nullptr;

@@ -76,7 +76,7 @@ class BatchStats(C.Structure):
     _fields_ = [("time_ms", C.c_double), ("waves", C.c_int32), ("levels", C.c_int32),
                 ("active_vertices", C.c_int64), ("active_edges", C.c_int64), ("touched_vertices", C.c_int64),
                 ("vw", C.c_int32), ("tile", C.c_int32), ("filters", C.c_int32), ("phases", C.c_int32),
-                ("launches", C.c_int64)]
+                ("launches", C.c_int64), ("bridges", C.c_int32), ("bridged", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

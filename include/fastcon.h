@@ -312,6 +312,9 @@ typedef struct fc_batch_stats {
   int32_t filters;             /* frontier-list filter passes (resolved queries dropped) */
   int32_t phases;              /* kernel launches that did work */
   int64_t launches;            /* kernel launches issued (phases + no-op tail launches) */
+  int32_t bridges;             /* bridge steps: edges between the two sides' frontiers tested
+                                  directly (heuristic and learned schedules) */
+  int32_t bridged;             /* queries a bridge step answered */
 } fc_batch_stats;
 
 /* Batched s–t: queries are run as bit-parallel waves of 64 (one uint64 mask

@@ -12,6 +12,7 @@ plain visit's virtual-warp width is fixed to 4, 8, 16, 32 or 64 on every
 other graph, and the grid varies between 1 and 4 blocks per CU. One more
 batched run per graph draws its width, hub chunk, lane count, launch mode
 and blocks per CU at random."""
+import os
 import sys
 
 import numpy as np
@@ -136,18 +137,31 @@ def test_random_family_matches_host_bfs(fc, orc, monkeypatch, seed):
             btile = int(rng.choice([64, 128, 256, 512, 1024, 4096]))
             blanes = int(rng.integers(1, 5))
             bgraph = int(rng.integers(0, 2))
-            say(f"random seed={seed} {name} batch vw={bvw} tile={btile} lanes={blanes} graph={bgraph} "
-                f"bpc={1 + k % 3}")
-            res = g.stcon_batch(s, t, policy=fc.POLICY_HEURISTIC, vw=bvw, tile=btile, use_graph=bgraph, lanes=blanes)
-            bad = np.flatnonzero(res.dist != exp)
-            assert bad.size == 0, (name, bvw, btile, blanes, bgraph, bad[:5], res.dist[bad[:5]], exp[bad[:5]])
-            w = rng.normal(0, 0.5, mlp.P).astype(np.float32)
-            g.sched_set_params(w)
-            for lanes in (1, 3):
-                say(f"random seed={seed} {name} batch learned (random weights) lanes={lanes}")
-                res = g.stcon_batch(s, t, policy=fc.POLICY_LEARNED, learn_margin=0.0, lanes=lanes)
+            # bridge steps before every eligible step, before large ones only,
+            # or (the default threshold) hardly ever on graphs this small
+            bmin = int(rng.choice([1, 64, 1 << 20]))
+            old_bmin = os.environ.get("FASTCON_BRIDGE_MIN_EDGES")
+            os.environ["FASTCON_BRIDGE_MIN_EDGES"] = str(bmin)
+            try:
+                say(f"random seed={seed} {name} batch vw={bvw} tile={btile} lanes={blanes} graph={bgraph} "
+                    f"bpc={1 + k % 3} bridge_min={bmin}")
+                res = g.stcon_batch(s, t, policy=fc.POLICY_HEURISTIC, vw=bvw, tile=btile, use_graph=bgraph,
+                                    lanes=blanes)
                 bad = np.flatnonzero(res.dist != exp)
-                assert bad.size == 0, (name, "learned", lanes, bad[:5], res.dist[bad[:5]], exp[bad[:5]])
+                assert bad.size == 0, (name, bvw, btile, blanes, bgraph, bmin, bad[:5], res.dist[bad[:5]],
+                                       exp[bad[:5]])
+                w = rng.normal(0, 0.5, mlp.P).astype(np.float32)
+                g.sched_set_params(w)
+                for lanes in (1, 3):
+                    say(f"random seed={seed} {name} batch learned (random weights) lanes={lanes} bridge_min={bmin}")
+                    res = g.stcon_batch(s, t, policy=fc.POLICY_LEARNED, learn_margin=0.0, lanes=lanes)
+                    bad = np.flatnonzero(res.dist != exp)
+                    assert bad.size == 0, (name, "learned", lanes, bmin, bad[:5], res.dist[bad[:5]], exp[bad[:5]])
+            finally:
+                if old_bmin is None:
+                    os.environ.pop("FASTCON_BRIDGE_MIN_EDGES", None)
+                else:
+                    os.environ["FASTCON_BRIDGE_MIN_EDGES"] = old_bmin
             for a, b, d in list(zip(s, t, exp))[2:6]:
                 say(f"random seed={seed} {name} seeds s={a} t={b}")
                 assert g.stcon_seeds(int(a), int(b), helpers=int(rng.integers(1, 64)),
