@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,6 +41,11 @@ namespace fcd {
 using fc::fail;
 
 constexpr uint32_t kBStage = 512;  // staged entries per wave and list
+// The control block's state: the words before the phase log, which the
+// finalising wave copies to LDS.
+constexpr uint32_t kCtlHotWords = offsetof(BatchCtl, plog) / sizeof(unsigned long long);
+static_assert(offsetof(BatchCtl, plog) % sizeof(unsigned long long) == 0, "state ends on a word boundary");
+static_assert(kCtlHotWords <= 2 * kWave, "two loads per lane at most");
 // Margin of the learned policy when the caller gives none: the value model
 // selection chose on the validation graphs for the shipped parameters
 // (fastcon/sched_report.json), so C callers, the CLIs and the Python binding
@@ -932,37 +938,37 @@ __device__ __forceinline__ int batch_next(BatchCtl* ctl, bool allow_filter) {
 
 // Counters a launch updated with atomics, as the finalisation needs them:
 // INIT's lists at parity 0 (and the live mask), the other phases' next
-// lists. Loaded by the whole finalising wave (uniform addresses) before the
-// slot sums, so both sets of loads share one round trip.
+// lists. Read from the finalising wave's copy of the control block, which
+// was loaded after every block had voted.
 struct FinPre {
   uint32_t ln[2];
   unsigned long long lh[2], lm[2], act0;
 };
 
-__device__ __forceinline__ FinPre fin_prefetch(BatchCtl* ctl, int phase, const int* nxt) {
+__device__ __forceinline__ FinPre fin_prefetch(const BatchCtl* ctl, int phase, const int* nxt) {
   FinPre f{{0u, 0u}, {0ull, 0ull}, {0ull, 0ull}, 0ull};
   if (phase == kBInit) {
-    f.act0 = ld_agent(&ctl->act);
+    f.act0 = ctl->act;
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      f.ln[x] = ld_agent(&ctl->fl_n[x][0]);
-      f.lh[x] = ld_agent(&ctl->hub[x][0]);
-      f.lm[x] = ld_agent(&ctl->fl_mf[x][0]);
+      f.ln[x] = ctl->fl_n[x][0];
+      f.lh[x] = ctl->hub[x][0];
+      f.lm[x] = ctl->fl_mf[x][0];
     }
   } else if (phase == kBCommit || phase == kBBottomUp || phase == kBFilter) {
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      f.ln[x] = ld_agent(&ctl->fl_n[x][nxt[x]]);
-      f.lh[x] = ld_agent(&ctl->hub[x][nxt[x]]);
+      f.ln[x] = ctl->fl_n[x][nxt[x]];
+      f.lh[x] = ctl->hub[x][nxt[x]];
     }
   }
   return f;
 }
 
+// ctl: the finalising wave's copy of the control block's state (LDS).
 template <typename OffT>
-__device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, int phase, const BatchDecision& d,
-                                              const RedTot& r, const FinPre& f) {
-  BatchCtl* ctl = p.ctl;
+__device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, BatchCtl* ctl, int phase,
+                                              const BatchDecision& d, const RedTot& r, const FinPre& f) {
   const int wave0 = ctl->wave * 64;
   const uint32_t ln[2] = {f.ln[0], f.ln[1]};
   const unsigned long long lh[2] = {f.lh[0], f.lh[1]}, lm[2] = {f.lm[0], f.lm[1]}, act0 = f.act0;
@@ -1339,24 +1345,39 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       __builtin_amdgcn_global_load_lds((gptr)src, (lptr)(s_mlp + k * kWave), 4, 0, 0);
     }
   }
+  // The state update and the decision run on an LDS copy of the control
+  // block's state (everything before the phase log), loaded with one
+  // agent-scope load per lane and written back with one store per lane: one
+  // lane's chain of read-modify-writes on global fields had cost a round
+  // trip each (loads and stores share one wait counter), 4-11 us per launch.
+  // Every other block has voted, so no one else writes these words now.
+  __shared__ __align__(16) unsigned long long s_ctl[kCtlHotWords];
+  {
+    const unsigned long long* gw = reinterpret_cast<const unsigned long long*>(ctl);
+    for (int k = lane_id(); k < (int)kCtlHotWords; k += kWave) s_ctl[k] = ld_agent(gw + k);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  BatchCtl* lc = reinterpret_cast<BatchCtl*>(s_ctl);
   int need = kFinDone;
   {
-    const FinPre pre = fin_prefetch(ctl, phase, nxt);
+    const FinPre pre = fin_prefetch(lc, phase, nxt);
     const RedTot tot = collect_slots(p.red);
     if (threadIdx.x == 0) {
-      if (ctl->phases < kPhaseLog) {
-        ctl->plog[ctl->phases] = (int8_t)(phase * 16 + (phase == kBExpand || phase == kBBottomUp ? d.sides : 0));
-        ctl->ptime[ctl->phases] = (int64_t)wall_clock64();
-        ctl->pwork[ctl->phases] = (int64_t)(phase == kBCommit ? tot.v[kRTouched] : tot.v[kRActE]);
+      const int64_t ph = lc->phases;
+      if (ph < kPhaseLog) {
+        ctl->plog[ph] = (int8_t)(phase * 16 + (phase == kBExpand || phase == kBBottomUp ? d.sides : 0));
+        ctl->ptime[ph] = (int64_t)wall_clock64();
+        ctl->pwork[ph] = (int64_t)(phase == kBCommit ? tot.v[kRTouched] : tot.v[kRActE]);
       }
-      ctl->phases++;
+      lc->phases = ph + 1;
       if (phase == kBDecide) {
-        need = ctl->pause_filter ? kFinDecideFilter : kFinDecide;
+        need = lc->pause_filter ? kFinDecideFilter : kFinDecide;
       } else {
-        need = batch_finalize(p, phase, d, tot, pre);
-        if (need != kFinDone && bridge_due(p, ctl)) {
-          ctl->bridge_lv = ((ctl->lvl[0] + 1) << 16) | (ctl->lvl[1] + 1);
-          ctl->phase = kBBridge;
+        need = batch_finalize(p, lc, phase, d, tot, pre);
+        if (need != kFinDone && bridge_due(p, lc)) {
+          lc->bridge_lv = ((lc->lvl[0] + 1) << 16) | (lc->lvl[1] + 1);
+          lc->phase = kBBridge;
           need = kFinDone;
         }
       }
@@ -1366,34 +1387,40 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   if (need != kFinDone) {
     float f[12];
     int a = 0;
-    if (threadIdx.x == 0) a = batch_heuristic(ctl, p.V, p.E, f);
+    if (threadIdx.x == 0) a = batch_heuristic(lc, p.V, p.E, f);
     a = __shfl(a, 0);
-    const bool forced = phase == kBDecide && ctl->force_action >= 0;
-    const bool pause = phase != kBDecide && ctl->trace &&
-                       ctl->decisions % (ctl->trace_every > 0 ? ctl->trace_every : 1) == 0;
+    const bool forced = phase == kBDecide && lc->force_action >= 0;
+    const bool pause = phase != kBDecide && lc->trace &&
+                       lc->decisions % (lc->trace_every > 0 ? lc->trace_every : 1) == 0;
     if (!forced && !pause && mlp_on) {
       __builtin_amdgcn_s_waitcnt(0);  // the weights are in LDS
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < 12; ++k) f[k] = __shfl(f[k], 0);
-      a = mlp_pick_wave_t(s_mlp, f, a, 1e-3f * (float)ctl->learn_margin_milli);
+      a = mlp_pick_wave_t(s_mlp, f, a, 1e-3f * (float)lc->learn_margin_milli);
     }
     if (threadIdx.x == 0) {
       if (pause) {
-        ctl->heur_action = a;
-        ctl->pause_filter = need == kFinDecideFilter;
-        ctl->phase = kBPause;
+        lc->heur_action = a;
+        lc->pause_filter = need == kFinDecideFilter;
+        lc->phase = kBPause;
       } else {
-        if (forced) a = ctl->force_action;
-        batch_apply(ctl, a, need == kFinDecideFilter);
-        ctl->decisions++;
+        if (forced) a = lc->force_action;
+        batch_apply(lc, a, need == kFinDecideFilter);
+        lc->decisions++;
       }
-      if (phase == kBDecide) ctl->t_decide = (int64_t)wall_clock64();
+      if (phase == kBDecide) lc->t_decide = (int64_t)wall_clock64();
     }
   }
   // no copy into LDS may still be in flight when the wave ends
   if (mlp_on) __builtin_amdgcn_s_waitcnt(0);
-  if (threadIdx.x == 0) __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) lc->ticket = 0u;
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  {
+    unsigned long long* gw = reinterpret_cast<unsigned long long*>(ctl);
+    for (int k = lane_id(); k < (int)kCtlHotWords; k += kWave) gw[k] = s_ctl[k];
+  }
 }
 
 template <typename OffT>

@@ -276,7 +276,13 @@ fc_status fc_validate_stcon(fc_graph* g, int32_t s, int32_t t, int32_t mn, int32
 fc_status fc_stcon_sched_many(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int32_t policy,
                               int32_t* connected, int32_t* dist, fc_stats* stats);
 
-/* Scheduling policy of a visit step. */
+/* Scheduling policy of a visit step. In fc_stcon_batch, the heuristic and
+ * learned policies also run bridge steps: before a side whose live frontier
+ * has at least 1 M edges (FASTCON_BRIDGE_MIN_EDGES) expands, the edges between
+ * the two sides' current frontiers are tested directly, which answers the
+ * queries they connect (distance = sum of the two levels + 1). Graphs whose
+ * device lists are not id-sorted (FC_GRAPH_KEEP_ORDER, FC_GRAPH_KEEP_IDS)
+ * run without them. Distances are the same under every policy. */
 typedef enum fc_policy {
   FC_POLICY_PLAIN = 0,      /* both colours, top-down, every iteration (reference semantics) */
   FC_POLICY_HEURISTIC = 1,  /* smaller side first; direction and width from frontier statistics */
