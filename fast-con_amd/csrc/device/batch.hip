@@ -140,23 +140,6 @@ struct RedTot {
   unsigned long long v[kRCount];
 };
 
-// Wave 0 of the finalising block: sum the slots and clear them. All loads
-// are issued before any is used.
-__device__ __forceinline__ RedTot collect_slots(unsigned long long* red) {
-  const int lane = lane_id();
-  unsigned long long v[kRCount];
-  unsigned long long* row = red + (size_t)(lane < kRedSlots ? lane : 0) * kRedWords;
-#pragma unroll
-  for (int k = 0; k < kRCount; ++k) v[k] = lane < kRedSlots ? ld_agent(row + k) : 0ull;
-  RedTot t;
-#pragma unroll
-  for (int k = 0; k < kRCount; ++k) {
-    if (v[k]) row[k] = 0ull;
-    t.v[k] = k < kRMfS ? (unsigned long long)wave_or_u64(v[k]) : (unsigned long long)wave_sum_u64(v[k]);
-  }
-  return t;
-}
-
 // Per-wave append state of the commit, bottom-up and filter phases: the next
 // frontier lists of both sides (normal vertices and hubs).
 struct Appends {
@@ -1351,18 +1334,33 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   // lane's chain of read-modify-writes on global fields had cost a round
   // trip each (loads and stores share one wait counter), 4-11 us per launch.
   // Every other block has voted, so no one else writes these words now.
+  // The state words and the launch's reduction slots are loaded together,
+  // in one round trip.
   __shared__ __align__(16) unsigned long long s_ctl[kCtlHotWords];
+  RedTot tot;
   {
+    const int lane = lane_id();
     const unsigned long long* gw = reinterpret_cast<const unsigned long long*>(ctl);
-    for (int k = lane_id(); k < (int)kCtlHotWords; k += kWave) s_ctl[k] = ld_agent(gw + k);
+    unsigned long long c0 = 0, c1 = 0;
+    if (lane < (int)kCtlHotWords) c0 = ld_agent(gw + lane);
+    if (lane + kWave < (int)kCtlHotWords) c1 = ld_agent(gw + lane + kWave);
+    unsigned long long* row = p.red + (size_t)(lane < kRedSlots ? lane : 0) * kRedWords;
+    unsigned long long v[kRCount];
+#pragma unroll
+    for (int k = 0; k < kRCount; ++k) v[k] = lane < kRedSlots ? ld_agent(row + k) : 0ull;
+    if (lane < (int)kCtlHotWords) s_ctl[lane] = c0;
+    if (lane + kWave < (int)kCtlHotWords) s_ctl[lane + kWave] = c1;
+#pragma unroll
+    for (int k = 0; k < kRCount; ++k) {
+      if (v[k]) row[k] = 0ull;
+      tot.v[k] = k < kRMfS ? (unsigned long long)wave_or_u64(v[k]) : (unsigned long long)wave_sum_u64(v[k]);
+    }
   }
-  __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
   BatchCtl* lc = reinterpret_cast<BatchCtl*>(s_ctl);
   int need = kFinDone;
   {
     const FinPre pre = fin_prefetch(lc, phase, nxt);
-    const RedTot tot = collect_slots(p.red);
     if (threadIdx.x == 0) {
       const int64_t ph = lc->phases;
       if (ph < kPhaseLog) {
