@@ -1894,7 +1894,7 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
 }
 
 // Autotuner: time the first waves of this batch under each (width, tile)
-// candidate and keep the fastest for this graph.
+// candidate (the median of three runs) and keep the fastest for this graph.
 static fc_status batch_autotune(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy) {
   BatchWorkspace* b = g->batch;
   const int32_t m = std::min<int32_t>(n, 128);
@@ -1904,10 +1904,15 @@ static fc_status batch_autotune(fc_graph* g, const int32_t* s, const int32_t* t,
   int bvw = 0, btile = 256;
   for (int vw : vws)
     for (int tile : tiles) {
-      float ms = 0;
-      BatchCtl c;
-      fc_status st = batch_run(g, s, t, m, policy, vw, tile, false, &ms, &c);
-      if (st != FC_OK) return st;
+      // median of three timings, so that one noisy run does not pick the pair
+      float ts[3];
+      for (float& x : ts) {
+        BatchCtl c;
+        fc_status st = batch_run(g, s, t, m, policy, vw, tile, false, &x, &c);
+        if (st != FC_OK) return st;
+      }
+      std::sort(ts, ts + 3);
+      const float ms = ts[1];
       if (ms < best) {
         best = ms;
         bvw = vw;
