@@ -859,6 +859,14 @@ __device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t
   for (int k = 0; k < 12; ++k) f[k] = ctl->feat[k];
   if (ctl->policy == 0) return 2;  // plain: both sides top-down
   if (mf[0] + mf[1] <= ctl->both_max_e) return 2;
+  // Right after a bridge step at these levels (i, j), no live query has
+  // distance i + j + 1 unless the bridge left pairs untested, so the next
+  // answers need both sides to advance: expanding one side first would only
+  // add a commit and a filter. Both sides, unless one frontier is large
+  // enough for bottom-up.
+  if (ctl->bridge_lv == (((ctl->lvl[0] + 1) << 16) | (ctl->lvl[1] + 1)) && (double)mf[0] * 4.0 <= (double)E &&
+      (double)mf[1] * 4.0 <= (double)E)
+    return 2;
   const int side = mf[0] <= mf[1] ? 0 : 1;
   // A bit-parallel bottom-up vertex stops only once it has found every live
   // query's bit: worth it only when the top-down frontier would touch a large
