@@ -67,29 +67,29 @@ template <typename OffT>
 static VisitParams<OffT> make_params(fc_graph* g) {
   VisitParams<OffT> p;
   std::memset(&p, 0, sizeof(p));
-  p.off = (const OffT*)g->d_off;
-  p.col = g->d_col;
+  set_param(p.off, (const OffT*)g->d_off);
+  set_param(p.col, g->d_col);
   p.V = (int32_t)g->V;
   p.E = g->E;
   p.hub_threshold = g->hub_threshold;
-  p.state = g->d_state;
-  p.sst = g->d_sst;
-  p.parent = g->d_parent;
-  p.visited = g->d_visited;
+  set_param(p.state, g->d_state);
+  set_param(p.sst, g->d_sst);
+  set_param(p.parent, g->d_parent);
+  set_param(p.visited, g->d_visited);
   for (int c = 0; c < 2; ++c) {
-    for (int k = 0; k < 3; ++k) p.fbm[c][k] = g->d_fbm[c][k];
+    for (int k = 0; k < 3; ++k) set_param(p.fbm[c][k], g->d_fbm[c][k]);
     for (int k = 0; k < 2; ++k) {
-      p.q[c][k] = g->d_q[c][k];
-      p.hq[c][k] = g->d_hq[c][k];
-      p.hb[c][k] = g->d_hb[c][k];
+      set_param(p.q[c][k], g->d_q[c][k]);
+      set_param(p.hq[c][k], g->d_hq[c][k]);
+      set_param(p.hb[c][k], g->d_hb[c][k]);
     }
   }
   p.qcap = g->qcap;
   p.hcap = g->hcap;
-  p.ctl = g->d_ctl;
-  p.qs = g->d_qs;
-  p.qt = g->d_qt;
-  p.out = g->d_out;
+  set_param(p.ctl, g->d_ctl);
+  set_param(p.qs, g->d_qs);
+  set_param(p.qt, g->d_qt);
+  set_param(p.out, g->d_out);
   p.lazy_state = g->relabeled ? 1 : 0;
   p.v_active = (int32_t)g->v_active;
   return p;

@@ -100,27 +100,27 @@ VisitParams<int32_t> static_params(unsigned char* base, const int32_t* off, cons
                                    int64_t E) {
   VisitParams<int32_t> p;
   std::memset(&p, 0, sizeof(p));
-  p.off = off;
-  p.col = col;
+  set_param(p.off, off);
+  set_param(p.col, col);
   p.V = V;
   p.hub_threshold = kStaticHub;
   p.E = E;
-  p.sst = (unsigned long long*)(base + kOffSst);
+  set_param(p.sst, (unsigned long long*)(base + kOffSst));
   // The plain s–t mode keeps no levels, parents or bitmaps (visit.hip):
   // those pointers stay null and are never dereferenced in this mode.
   for (int c = 0; c < 2; ++c)
     for (int k = 0; k < 2; ++k) {
       const int i = c * 2 + k;
-      p.q[c][k] = (int32_t*)(base + kOffQ + (size_t)i * al256(sizeof(int32_t) * kStaticMaxV));
-      p.hq[c][k] = (int32_t*)(base + kOffHq + (size_t)i * al256(sizeof(int32_t) * kStaticHubCap));
-      p.hb[c][k] = (int64_t*)(base + kOffHb + (size_t)i * al256(sizeof(int64_t) * kStaticHubCap));
+      set_param(p.q[c][k], (int32_t*)(base + kOffQ + (size_t)i * al256(sizeof(int32_t) * kStaticMaxV)));
+      set_param(p.hq[c][k], (int32_t*)(base + kOffHq + (size_t)i * al256(sizeof(int32_t) * kStaticHubCap)));
+      set_param(p.hb[c][k], (int64_t*)(base + kOffHb + (size_t)i * al256(sizeof(int64_t) * kStaticHubCap)));
     }
   p.qcap = (uint32_t)V;
   p.hcap = (uint32_t)kStaticHubCap;
-  p.ctl = (VisitCtl*)(base + kOffCtl);
-  p.qs = (const int32_t*)(base + kOffQs);
-  p.qt = (const int32_t*)(base + kOffQt);
-  p.out = (VisitOut*)(base + kOffOut);
+  set_param(p.ctl, (VisitCtl*)(base + kOffCtl));
+  set_param(p.qs, (const int32_t*)(base + kOffQs));
+  set_param(p.qt, (const int32_t*)(base + kOffQt));
+  set_param(p.out, (VisitOut*)(base + kOffOut));
   p.lazy_state = 0;
   p.v_active = V;
   return p;

@@ -247,23 +247,75 @@ __device__ __forceinline__ void propose(Appender<OffT>& a, int32_t dist, int32_t
   a.best = cand < a.best ? cand : a.best;
 }
 
-// Probe edge (v -> u) for colour c whose frontier is at level lvl. Returns
-// true when this lane claimed u for the next frontier; a meeting edge lowers
-// the lane's best candidate.
+// Top-down probe of edge (v -> u) for colour c whose frontier is at level
+// lvl, in three parts, so that a lane has the loads and the atomics of
+// several edges in flight together:
+//   probe_look   loads u's visited-bitmap word (BFS) or state word (s–t);
+//   probe_issue  when that word leaves u unlabelled, issues the claiming
+//                atomic together with the loads of u's offsets, so that a
+//                claimed vertex's degree arrives with the atomic's result
+//                instead of one round trip later;
+//   probe_done   returns whether this lane claimed u (with its degree); for
+//                s–t a meeting edge lowers the lane's best candidate.
 template <typename OffT>
-__device__ __forceinline__ bool probe_td(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int32_t v, int32_t u,
-                                         bool act) {
-  if (!act) return false;
+struct Probe {
+  unsigned long long w;    // the looked-up word
+  unsigned long long old;  // the atomic's result
+  OffT ob, oe;             // u's offsets (when the atomic was issued)
+  bool tried;              // the atomic was issued
+};
+
+template <typename OffT>
+__device__ __forceinline__ Probe<OffT> probe_look(const Appender<OffT>& a, bool st, int32_t u, bool act) {
   const VisitParams<OffT>& p = *a.p;
+  Probe<OffT> r;
+  r.w = 0;
+  r.old = 0;
+  r.ob = r.oe = 0;
+  r.tried = false;
+  if (!act) return r;
+  if (!st)
+    r.w = p.visited[u >> 5];
+  else if (u != a.root_own && u != a.root_other)
+    r.w = p.sst[u];
+  return r;
+}
+
+template <typename OffT>
+__device__ __forceinline__ void probe_issue(const Appender<OffT>& a, bool st, int lvl, int32_t u, bool act,
+                                            Probe<OffT>& r) {
+  const VisitParams<OffT>& p = *a.p;
+  if (!act) return;
   if (!st) {
     // BFS claims through the visited bitmap: one atomicOr per newly reached
-    // vertex; the plain pre-load filters most already-visited probes.
+    // vertex; the looked-up word filters most already-visited probes.
     const uint32_t bit = 1u << (u & 31);
-    uint32_t* w = p.visited + (u >> 5);
-    if (*w & bit) return false;
-    if (atomicOr(w, bit) & bit) return false;
+    if ((uint32_t)r.w & bit) return;
+    r.ob = p.off[u];
+    r.oe = p.off[u + 1];
+    r.old = atomicOr(p.visited + (u >> 5), bit);
+    r.tried = true;
+    return;
+  }
+  // A word of another epoch is unvisited.
+  if (u == a.root_own || u == a.root_other || (uint32_t)(r.w >> 32) == a.epoch) return;
+  r.ob = p.off[u];
+  r.oe = p.off[u + 1];
+  r.old = atomicCAS(&p.sst[u], r.w, st_word(a.epoch, a.c, lvl + 1));
+  r.tried = true;
+}
+
+template <typename OffT>
+__device__ __forceinline__ bool probe_done(Appender<OffT>& a, int32_t* parent, bool st, int lvl, int32_t v,
+                                           int32_t u, bool act, const Probe<OffT>& r, int64_t& deg) {
+  const VisitParams<OffT>& p = *a.p;
+  deg = 0;
+  if (!act) return false;
+  if (!st) {
+    if (!r.tried || ((uint32_t)r.old & (1u << (u & 31)))) return false;
     p.state[u] = lvl + 1;
     if (parent) parent[u] = v;
+    deg = (int64_t)(r.oe - r.ob);
     return true;
   }
   if (u == a.root_own) return false;
@@ -271,18 +323,24 @@ __device__ __forceinline__ bool probe_td(Appender<OffT>& a, int32_t* parent, boo
     propose(a, lvl + 1, u);
     return false;
   }
-  const unsigned long long mine = st_word(a.epoch, a.c, lvl + 1);
-  unsigned long long cur = p.sst[u];
-  // A word of another epoch is unvisited. The plain load may be stale, so a
-  // failed CAS retries with the value it returns; within a launch the word
-  // only changes to this epoch, so the second attempt settles it.
-  for (int k = 0; k < 3 && (uint32_t)(cur >> 32) != a.epoch; ++k) {
-    const unsigned long long old = atomicCAS(&p.sst[u], cur, mine);
-    if (old == cur) {
+  unsigned long long cur = r.w;
+  if (r.tried) {
+    // The looked-up word may be stale, so a failed CAS retries with the value
+    // it returns; within a launch the word only changes to this epoch, so
+    // the second attempt settles it (three attempts in all).
+    const unsigned long long mine = st_word(a.epoch, a.c, lvl + 1);
+    cur = r.old;
+    bool won = r.old == r.w;
+    for (int k = 0; k < 2 && !won && (uint32_t)(cur >> 32) != a.epoch; ++k) {
+      const unsigned long long old = atomicCAS(&p.sst[u], cur, mine);
+      won = old == cur;
+      cur = old;
+    }
+    if (won) {
       if (parent) parent[u] = v;
+      deg = (int64_t)(r.oe - r.ob);
       return true;
     }
-    cur = old;
   }
   if ((uint32_t)(cur >> 32) != a.epoch) {
     atomicOr(&p.ctl->err, 4u);
@@ -291,11 +349,6 @@ __device__ __forceinline__ bool probe_td(Appender<OffT>& a, int32_t* parent, boo
   const int32_t w = (int32_t)(uint32_t)cur;
   if (((w & kColorBit) != 0) != (a.c != 0)) propose(a, lvl + 1 + (w & kLevelMask), u);
   return false;
-}
-
-template <typename OffT>
-__device__ __forceinline__ int64_t degree_of(const VisitParams<OffT>& p, int32_t u) {
-  return (int64_t)(p.off[u + 1] - p.off[u]);
 }
 
 template <typename OffT, int VW>
@@ -326,10 +379,14 @@ __device__ __forceinline__ void td_normal(Appender<OffT>& a, int32_t* parent, bo
       const bool a0 = b < e, a1 = b + VW < e;
       const int32_t u0 = a0 ? p.col[b] : 0;
       const int32_t u1 = a1 ? p.col[b + VW] : 0;
-      const bool c0 = probe_td(a, parent, st, lvl, v, u0, a0);
-      const bool c1 = probe_td(a, parent, st, lvl, v, u1, a1);
-      push_next(a, c0, u0, c0 ? degree_of(p, u0) : 0, a.bitmaps);
-      push_next(a, c1, u1, c1 ? degree_of(p, u1) : 0, a.bitmaps);
+      Probe<OffT> r0 = probe_look(a, st, u0, a0), r1 = probe_look(a, st, u1, a1);
+      probe_issue(a, st, lvl, u0, a0, r0);
+      probe_issue(a, st, lvl, u1, a1, r1);
+      int64_t d0, d1;
+      const bool c0 = probe_done(a, parent, st, lvl, v, u0, a0, r0, d0);
+      const bool c1 = probe_done(a, parent, st, lvl, v, u1, a1, r1, d1);
+      push_next(a, c0, u0, d0, a.bitmaps);
+      push_next(a, c1, u1, d1, a.bitmaps);
       b += 2 * VW;
     }
   }
@@ -394,8 +451,11 @@ __device__ __forceinline__ void td_hubs(Appender<OffT>& a, int32_t* parent, bool
         }
       }
       const bool ok = act && v >= 0;
-      const bool cl = probe_td(a, parent, st, lvl, v, u, ok);
-      push_next(a, cl, u, cl ? degree_of(p, u) : 0, a.bitmaps);
+      Probe<OffT> r = probe_look(a, st, u, ok);
+      probe_issue(a, st, lvl, u, ok, r);
+      int64_t d;
+      const bool cl = probe_done(a, parent, st, lvl, v, u, ok, r, d);
+      push_next(a, cl, u, d, a.bitmaps);
     }
   }
 }
@@ -410,8 +470,11 @@ __device__ __forceinline__ void td_root(Appender<OffT>& a, int32_t* parent, int3
     const OffT j = k + (OffT)lane_id();
     const bool act = j < e;
     const int32_t u = act ? p.col[j] : 0;
-    const bool cl = probe_td(a, parent, true, 0, r, u, act);
-    push_next(a, cl, u, cl ? degree_of(p, u) : 0, false);
+    Probe<OffT> pr = probe_look(a, true, u, act);
+    probe_issue(a, true, 0, u, act, pr);
+    int64_t d;
+    const bool cl = probe_done(a, parent, true, 0, r, u, act, pr, d);
+    push_next(a, cl, u, d, false);
   }
 }
 

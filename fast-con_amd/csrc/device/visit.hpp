@@ -75,6 +75,24 @@ struct VisitState {
 };
 static_assert(sizeof(VisitState) == 128, "VisitState layout");
 
+// Pointers into device memory. In the device pass they carry the global
+// address space: the kernel reads them from its LDS copy of the control
+// block, and without it the compiler cannot tell them from LDS pointers. It
+// then emits flat instructions, each of which also counts as an LDS access
+// (every wait for one waits for the LDS too), and reloads the pointers from
+// LDS after each flat store, since such a store might have written there.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FC_GLOBAL __attribute__((address_space(1)))
+#else
+#define FC_GLOBAL
+#endif
+
+// Host code stores a device pointer into such a field with an explicit cast.
+template <typename F, typename T>
+inline void set_param(F& field, T* v) {
+  field = (F)v;
+}
+
 // Launch parameters: the graph and workspace pointers. They travel inside
 // the control block (the kernel's only pointer argument), so that every
 // block gets them with the same 1 KB load as the machine state instead of
@@ -82,25 +100,25 @@ static_assert(sizeof(VisitState) == 128, "VisitState layout");
 // lazily and waits for one by one.
 template <typename OffT>
 struct VisitParams {
-  const OffT* off;
-  const int32_t* col;
+  FC_GLOBAL const OffT* off;
+  FC_GLOBAL const int32_t* col;
   int32_t V;
   int32_t hub_threshold;
   int64_t E;
-  int32_t* state;           // BFS: levels
-  unsigned long long* sst;  // s–t: (epoch << 32) | colour | level; another epoch = unvisited
-  int32_t* parent;
-  uint32_t* visited;        // ceil(V/64)*2 words
-  uint32_t* fbm[2][3];      // frontier bitmaps, per colour, ring of 3
-  int32_t* q[2][2];         // normal queues, per colour and parity
-  int32_t* hq[2][2];        // hub-queue vertices
-  int64_t* hb[2][2];        // hub-queue edge bases (ascending with slot)
+  FC_GLOBAL int32_t* state;           // BFS: levels
+  FC_GLOBAL unsigned long long* sst;  // s–t: (epoch << 32) | colour | level; another epoch = unvisited
+  FC_GLOBAL int32_t* parent;
+  FC_GLOBAL uint32_t* visited;        // ceil(V/64)*2 words
+  FC_GLOBAL uint32_t* fbm[2][3];      // frontier bitmaps, per colour, ring of 3
+  FC_GLOBAL int32_t* q[2][2];         // normal queues, per colour and parity
+  FC_GLOBAL int32_t* hq[2][2];        // hub-queue vertices
+  FC_GLOBAL int64_t* hb[2][2];        // hub-queue edge bases (ascending with slot)
   uint32_t qcap;
   uint32_t hcap;
-  VisitCtl* ctl;
-  const int32_t* qs;        // query sources (device), nq entries
-  const int32_t* qt;        // query targets
-  VisitOut* out;            // results (device), nq entries
+  FC_GLOBAL VisitCtl* ctl;
+  FC_GLOBAL const int32_t* qs;        // query sources (device), nq entries
+  FC_GLOBAL const int32_t* qt;        // query targets
+  FC_GLOBAL VisitOut* out;            // results (device), nq entries
   int32_t lazy_state;       // BFS leaves unreached state/parent unwritten (renumbered graphs:
                             // the un-permutation takes reached-ness from the visited bitmap)
   int32_t v_active;         // vertices [0, v_active) have degree > 0 (the rest are isolated)
