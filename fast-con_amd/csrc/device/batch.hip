@@ -851,12 +851,16 @@ __device__ __forceinline__ bool bridge_due(const BatchParams<OffT>& p, const Bat
 }
 
 // Heuristic (or plain) action for the pending step; also fills the feature
-// vector f for the learned policy. One lane of the finalising wave.
-__device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t E, float* f) {
+// vector f when the learned policy or a trace needs it (double-precision
+// logarithms on one lane: not worth computing otherwise). One lane of the
+// finalising wave.
+__device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t E, float* f, bool feats) {
   uint64_t mf[2];
   for (int x = 0; x < 2; ++x) mf[x] = live_mf(ctl, x);
-  batch_features(ctl, mf, V, E);
-  for (int k = 0; k < 12; ++k) f[k] = ctl->feat[k];
+  if (feats) {
+    batch_features(ctl, mf, V, E);
+    for (int k = 0; k < 12; ++k) f[k] = ctl->feat[k];
+  }
   if (ctl->policy == 0) return 2;  // plain: both sides top-down
   if (mf[0] + mf[1] <= ctl->both_max_e) return 2;
   // Right after a bridge step at these levels (i, j), no live query has
@@ -1391,9 +1395,9 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   }
   need = __shfl(need, 0);
   if (need != kFinDone) {
-    float f[12];
+    float f[12] = {};
     int a = 0;
-    if (threadIdx.x == 0) a = batch_heuristic(lc, p.V, p.E, f);
+    if (threadIdx.x == 0) a = batch_heuristic(lc, p.V, p.E, f, mlp_on || lc->trace);
     a = __shfl(a, 0);
     const bool forced = phase == kBDecide && lc->force_action >= 0;
     const bool pause = phase != kBDecide && lc->trace &&
