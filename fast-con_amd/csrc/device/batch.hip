@@ -832,7 +832,7 @@ __device__ __forceinline__ uint64_t live_mf(const BatchCtl* ctl, int x) {
   const int built = __popcll(ctl->lact[x]);
   const int live = __popcll(ctl->lact[x] & ctl->act);
   if (built == 0 || live == built) return mf;
-  return (uint64_t)((double)mf * (double)live / (double)built);
+  return mf * (uint64_t)live / (uint64_t)built;  // mf < 2^40 and live <= 64: no overflow
 }
 
 template <typename OffT>
@@ -868,14 +868,14 @@ __device__ __forceinline__ int batch_heuristic(BatchCtl* ctl, int64_t V, int64_t
   // answers need both sides to advance: expanding one side first would only
   // add a commit and a filter. Both sides, unless one frontier is large
   // enough for bottom-up.
-  if (ctl->bridge_lv == (((ctl->lvl[0] + 1) << 16) | (ctl->lvl[1] + 1)) && (double)mf[0] * 4.0 <= (double)E &&
-      (double)mf[1] * 4.0 <= (double)E)
+  if (ctl->bridge_lv == (((ctl->lvl[0] + 1) << 16) | (ctl->lvl[1] + 1)) && mf[0] * 4 <= (uint64_t)E &&
+      mf[1] * 4 <= (uint64_t)E)
     return 2;
   const int side = mf[0] <= mf[1] ? 0 : 1;
   // A bit-parallel bottom-up vertex stops only once it has found every live
   // query's bit: worth it only when the top-down frontier would touch a large
   // part of the graph anyway.
-  return side + ((double)mf[side] * 4.0 > (double)E ? 6 : 0);
+  return side + (mf[side] * 4 > (uint64_t)E ? 6 : 0);
 }
 
 // Turn action a into the step decision and the next phase, filtering first
@@ -900,14 +900,15 @@ __device__ __forceinline__ void batch_apply(BatchCtl* ctl, int a, bool allow_fil
     for (int x = 0; x < 2; ++x) {
       if (!(d.sides & (1 << x))) continue;
       const int c = ctl->cur[x];
-      // expected work saved: the share of the list's queries resolved since
-      // it was built
-      const int built = __popcll(ctl->lact[x]);
-      const int dead = __popcll(ctl->lact[x] & ~ctl->act);
+      // expected work saved: the share dead / built of the list's queries
+      // resolved since it was built (integer products: the lane-0 decision
+      // sits on every decision launch's critical path, and double-precision
+      // divisions there are slow)
+      const uint64_t built = (uint64_t)__popcll(ctl->lact[x]);
+      const uint64_t dead = (uint64_t)__popcll(ctl->lact[x] & ~ctl->act);
       if (!dead || !built) continue;
-      const double frac = (double)dead / (double)built;
-      const double size = (double)ctl->fl_n[x][c] + (double)(ctl->hub[x][c] >> kHubEdgeBits);
-      if (size * frac >= (double)ctl->filt_min_v || (double)ctl->fl_mf[x][c] * frac >= (double)ctl->filt_min_e)
+      const uint64_t size = (uint64_t)ctl->fl_n[x][c] + (ctl->hub[x][c] >> kHubEdgeBits);
+      if (size * dead >= ctl->filt_min_v * built || ctl->fl_mf[x][c] * dead >= ctl->filt_min_e * built)
         filt |= 1 << x;
     }
   }
@@ -1733,9 +1734,12 @@ fc_status batch_begin(fc_graph* g, BatchWorkspace* b, int32_t n, int policy, int
   init.trace_every = every;
   init.learn_margin_milli = (int32_t)std::lround((margin < 0.f ? kDefaultLearnMargin : margin) * 1000.f);
   init.f_branch = g->f_branch;
+  // Thresholds are capped at 2^56 (beyond any graph's counts), so that the
+  // decision's products with query counts (at most 64) cannot overflow.
   auto env_u64 = [](const char* name, uint64_t dflt) {
     const char* v = std::getenv(name);
-    return v && std::strtoull(v, nullptr, 10) > 0 ? (uint64_t)std::strtoull(v, nullptr, 10) : dflt;
+    const uint64_t x = v && std::strtoull(v, nullptr, 10) > 0 ? (uint64_t)std::strtoull(v, nullptr, 10) : dflt;
+    return std::min<uint64_t>(x, 1ull << 56);
   };
   init.filt_min_v = env_u64("FASTCON_FILTER_MIN_VERTICES", kFilterMinVertices);
   init.filt_min_e = env_u64("FASTCON_FILTER_MIN_EDGES", kFilterMinEdges);
