@@ -533,25 +533,19 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
     const bool small = cand && (e - b) <= (OffT)kBuLane;
     int32_t par = -1;
     if (small) {
-      // eight independent column loads per round keep requests in flight;
-      // most candidates find a parent in the first round
+      // eight independent column loads per round, then the eight bitmap
+      // words together, so a round costs two round trips whichever column
+      // holds the parent; most candidates find one in the first round
       for (OffT j = b; j < e && par < 0; j += 8) {
-        const int32_t u0 = p.col[j];
-        const int32_t u1 = j + 1 < e ? p.col[j + 1] : -1;
-        const int32_t u2 = j + 2 < e ? p.col[j + 2] : -1;
-        const int32_t u3 = j + 3 < e ? p.col[j + 3] : -1;
-        const int32_t u4 = j + 4 < e ? p.col[j + 4] : -1;
-        const int32_t u5 = j + 5 < e ? p.col[j + 5] : -1;
-        const int32_t u6 = j + 6 < e ? p.col[j + 6] : -1;
-        const int32_t u7 = j + 7 < e ? p.col[j + 7] : -1;
-        if (test_bit(cfbm, u0)) par = u0;
-        else if (u1 >= 0 && test_bit(cfbm, u1)) par = u1;
-        else if (u2 >= 0 && test_bit(cfbm, u2)) par = u2;
-        else if (u3 >= 0 && test_bit(cfbm, u3)) par = u3;
-        else if (u4 >= 0 && test_bit(cfbm, u4)) par = u4;
-        else if (u5 >= 0 && test_bit(cfbm, u5)) par = u5;
-        else if (u6 >= 0 && test_bit(cfbm, u6)) par = u6;
-        else if (u7 >= 0 && test_bit(cfbm, u7)) par = u7;
+        int32_t u[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) u[k] = j + k < e ? p.col[j + k] : -1;
+        uint32_t fw[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) fw[k] = u[k] >= 0 ? cfbm[u[k] >> 5] : 0u;
+#pragma unroll
+        for (int k = 7; k >= 0; --k)
+          if (u[k] >= 0 && ((fw[k] >> (u[k] & 31)) & 1u)) par = u[k];
       }
     }
     uint64_t big = ballot(cand && !small);
