@@ -1911,9 +1911,12 @@ static fc_status batch_run(fc_graph* g, const int32_t* s, const int32_t* t, int3
 
 // Autotuner: time the first waves of this batch under each (width, tile)
 // candidate (the median of three runs) and keep the fastest for this graph.
-static fc_status batch_autotune(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy) {
+static fc_status batch_autotune(fc_graph* g, const int32_t* s, const int32_t* t, int32_t n, int policy, int lanes) {
   BatchWorkspace* b = g->batch;
-  const int32_t m = std::min<int32_t>(n, 128);
+  // Eight waves, run as the batch itself runs (concurrent lanes, replayed
+  // graphs): two waves on one lane with direct launches had the tile choice
+  // flip from run to run.
+  const int32_t m = std::min<int32_t>(n, 512);
   const int vws[] = {0, 8, 32, 64};
   const int tiles[] = {256, 1024};
   float best = 1e30f;
@@ -1924,7 +1927,7 @@ static fc_status batch_autotune(fc_graph* g, const int32_t* s, const int32_t* t,
       float ts[3];
       for (float& x : ts) {
         BatchCtl c;
-        fc_status st = batch_run(g, s, t, m, policy, vw, tile, false, &x, &c);
+        fc_status st = batch_run(g, s, t, m, policy, vw, tile, true, &x, &c, nullptr, -1.f, lanes);
         if (st != FC_OK) return st;
       }
       std::sort(ts, ts + 3);
@@ -2005,7 +2008,7 @@ extern "C" fc_status fc_stcon_batch(fc_graph* g, const int32_t* s, const int32_t
   int vw = o.vw, tile = o.tile;
   if (vw == 0 && tile == 0) {
     if (b->tuned_vw < 0) {
-      st = fcd::batch_autotune(g, s, t, n, o.policy);
+      st = fcd::batch_autotune(g, s, t, n, o.policy, lanes);
       if (st != FC_OK) return st;
     }
     vw = b->tuned_vw;
