@@ -1,6 +1,6 @@
 """Train the batched engine's learned scheduler and report what it gains.
 
-    python -m fastcon.train_sched [--out DIR] [--quick]
+    python -m fastcon.train_sched [--out DIR] [--quick] [--evaluate]
 
 1. Trace: on a seeded family of graphs (undirected R-MAT with the initiator
    a varied around 0.57 and edge factors 8 and 16, Erdős–Rényi with mean
@@ -138,6 +138,28 @@ def end_to_end(g, s, t, w, margin):
     return out
 
 
+def evaluate(out, held):
+    """Re-measure the shipped network's end-to-end comparison on the held-out
+    graphs and queries with the engine as it is now, without tracing or
+    training again; the regret rows stay those of the training run."""
+    w = np.load(os.path.join(out, "sched_params.npy"), allow_pickle=False)
+    path = os.path.join(out, "sched_report.json")
+    with open(path) as fh:
+        report = json.load(fh)
+    margin = report["margin"]
+    for i, (name, build, pairs, every, cap) in enumerate(held):
+        g, hg = open_graph(build, 250 + i)
+        r2 = hg.sample_roots(2 * pairs, 300 + i)
+        report["end_to_end"][name] = end_to_end(g, r2[:pairs], r2[pairs:], w, margin)
+        log(f"  {name}: end to end {json.dumps(report['end_to_end'][name])}")
+        g.close()
+    report["end_to_end_note"] = ("re-measured by --evaluate with the engine of the present build; the regret rows "
+                                 "come from the training run's traces")
+    with open(path, "w") as fh:
+        json.dump(report, fh, indent=1)
+    log(f"updated {path}")
+
+
 def pick_actions(pred, heur, margin):
     """The learned policy's choice: the network's best action when it is
     predicted cheaper than the heuristic's by more than margin."""
@@ -156,8 +178,13 @@ def main():
     ap.add_argument("--repeats", type=int, default=3, help="timed rollouts per traced state (median)")
     ap.add_argument("--epochs-grid", default="200,600,2000",
                     help="final-fit epoch counts tried; the one with the lowest validation regret is kept")
+    ap.add_argument("--evaluate", action="store_true",
+                    help="only re-measure the shipped parameters end to end on the held-out graphs")
     args = ap.parse_args()
     train, valid, held = family(args.quick)
+    if args.evaluate:
+        evaluate(args.out, held)
+        return
 
     log("trace (training family, heuristic schedule)")
     F, Cst, Hh = [], [], []
