@@ -1261,9 +1261,25 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       break;
     }
     case kBCleanup: {
-      // Every vertex with a non-zero record or frontier mask is dirty: clear
-      // the records and all four mask arrays of the dirty vertices in
-      // address order.
+      // The only frontier masks still set are those of each side's current
+      // lists (an expansion clears the masks it reads, a commit its hub
+      // masks, a filter and kBClearF the lists they consume): clear them
+      // through the lists. Every vertex with a non-zero record is dirty:
+      // clear the records of the dirty vertices in address order.
+      for (int x = 0; x < 2; ++x) {
+        const int c = ctl->cur[x];
+        const uint32_t n = ctl->fl_n[x][c] < (uint32_t)p.V ? ctl->fl_n[x][c] : (uint32_t)p.V;
+        for (int64_t i = gtid; i < n; i += nthr) {
+          const int32_t v = p.fl[x][c][i];
+          if ((uint32_t)v < (uint32_t)p.V) p.F[x][c][v] = 0ull;
+        }
+        uint32_t nh = (uint32_t)(ctl->hub[x][c] >> kHubEdgeBits);
+        if (nh > p.hcap) nh = p.hcap;
+        for (int64_t i = gtid; i < nh; i += nthr) {
+          const int32_t v = p.hl[x][c][i];
+          if ((uint32_t)v < (uint32_t)p.V) p.F[x][c][v] = 0ull;
+        }
+      }
       const int lane = lane_id();
       const int64_t per = owned_groups(((int64_t)p.V + 31) / 32, gwave, nwaves);
       for (int64_t jb = 0; jb < per; jb += kWave) {
@@ -1286,10 +1302,6 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
           if (mine >= 0 && ((wv >> (lane & 31)) & 1u) && u < p.V) {
             *reinterpret_cast<ulonglong2*>(rec_of(p, 0, u)) = make_ulonglong2(0ull, 0ull);
             *reinterpret_cast<ulonglong2*>(rec_of(p, 1, u)) = make_ulonglong2(0ull, 0ull);
-            p.F[0][0][u] = 0ull;
-            p.F[0][1][u] = 0ull;
-            p.F[1][0][u] = 0ull;
-            p.F[1][1][u] = 0ull;
           }
         }
       }
