@@ -650,8 +650,8 @@ __device__ __forceinline__ void filter_side(const BatchParams<OffT>& p, Appends&
 // rule). The pairs come from the two frontier lists: tiles of 64 x 64 list
 // entries, one tile per wave at a time, with side t's entry broadcast
 // across the lanes; pairs with a common live bit are queued in the wave's
-// LDS and tested 64 at a time, each lane with a binary search of the
-// shorter id-sorted list. At R-MAT s22 about two thirds of the queries end
+// LDS and tested together, each lane searching the shorter id-sorted list of
+// up to four pairs at once. At R-MAT s22 about two thirds of the queries end
 // at distance 3, and the level-1 hubs that the next expansion would scan
 // (millions of edges) belong almost only to them: the queries that survive
 // a bridge carry about 5% of those edges. A pair left untested only leaves
@@ -664,36 +664,89 @@ constexpr uint64_t kBridgeMinEdges = 1ull << 20;   // live frontier edges of the
 static_assert(kBridgeQueue * 16 * kWavesPerBlock <= 2 * kWavesPerBlock * kBStage * sizeof(int32_t),
               "bridge queues fit the appends' stage");
 
-// Whether x is in y's adjacency list, which is in ascending id order.
-template <typename OffT>
-__device__ __forceinline__ bool sorted_has(const BatchParams<OffT>& p, int32_t y, int32_t x) {
-  OffT lo = p.off[y];
-  const OffT end = p.off[y + 1];
-  OffT hi = end;
-  while (lo < hi) {
-    const OffT mid = lo + (hi - lo) / 2;
-    if (p.col[mid] < x)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo < end && p.col[lo] == x;
-}
-
 // Test the queued pairs (wave-collective): OR the masks of adjacent pairs
-// into meets; each lane searches the list of the pair's lower-degree end.
+// into meets. Each lane takes the pairs lane, lane + 64, ... of the queue
+// together, and searches the id-sorted list of each pair's lower-degree end
+// for the other end. A round samples kSearchWays entries of every open range
+// at once, so it costs one memory round trip for all of the lane's pairs and
+// shrinks each range about kSearchWays times: a list of n entries takes
+// about log8(n) rounds, where a binary search took log2(n) rounds of one
+// dependent load each. The level-1 frontiers of R-MAT queries are mostly
+// hubs, and the lower degree of a tested pair is typically a few hundred.
+constexpr int kSearchWays = 8;
+
 template <typename OffT>
 __device__ __forceinline__ void bridge_flush(const BatchParams<OffT>& p, const int32_t* qa, const int32_t* qb,
                                              const unsigned long long* qm, uint32_t cnt,
                                              unsigned long long& meets) {
+  // pairs a lane searches at once: the whole queue for 32-bit offsets
+  constexpr int P = sizeof(OffT) == 4 ? (int)(kBridgeQueue / kWave) : 2;
   __builtin_amdgcn_wave_barrier();
-  for (uint32_t base = 0; base < cnt; base += kWave) {
-    const uint32_t k = base + (uint32_t)lane_id();
-    if (k < cnt) {
-      const int32_t a = qa[k], b = qb[k];
-      const OffT da = p.off[a + 1] - p.off[a], db = p.off[b + 1] - p.off[b];
-      if (da <= db ? sorted_has(p, a, b) : sorted_has(p, b, a)) meets |= qm[k];
+  const uint32_t lane = (uint32_t)lane_id();
+  for (uint32_t base = 0; base < cnt; base += (uint32_t)P * kWave) {
+    OffT lo[P], hi[P];
+    int32_t key[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+      const uint32_t k = base + (uint32_t)r * kWave + lane;
+      const bool on = k < cnt;
+      const int32_t a = on ? qa[k] : 0, b = on ? qb[k] : 0;
+      const OffT a0 = p.off[a], a1 = p.off[a + 1], b0 = p.off[b], b1 = p.off[b + 1];
+      const bool sa = a1 - a0 <= b1 - b0;
+      lo[r] = sa ? a0 : b0;
+      hi[r] = on ? (sa ? a1 : b1) : lo[r];
+      key[r] = sa ? b : a;
     }
+    uint32_t found = 0;
+    // Every round at least divides each open range by kSearchWays and a
+    // range of at most kSearchWays entries is read whole, so the loop ends.
+    while (true) {
+      bool open = false;
+#pragma unroll
+      for (int r = 0; r < P; ++r) open |= lo[r] < hi[r];
+      if (!ballot(open)) break;
+      int32_t v[P][kSearchWays];
+      OffT step[P];
+#pragma unroll
+      for (int r = 0; r < P; ++r) {
+        const OffT n = hi[r] - lo[r];
+        step[r] = n <= kSearchWays ? (OffT)1 : (n + kSearchWays - 1) / kSearchWays;
+        if (n > 0) {
+#pragma unroll
+          for (int i = 0; i < kSearchWays; ++i) {
+            const OffT pos = lo[r] + (OffT)i * step[r];
+            v[r][i] = p.col[pos < hi[r] ? pos : hi[r] - 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < P; ++r) {
+        if (lo[r] >= hi[r]) continue;
+        int c = 0;
+        bool hit = false;
+#pragma unroll
+        for (int i = 0; i < kSearchWays; ++i) {
+          if (lo[r] + (OffT)i * step[r] < hi[r]) {
+            hit |= v[r][i] == key[r];
+            c += v[r][i] < key[r] ? 1 : 0;
+          }
+        }
+        if (hit) found |= 1u << r;
+        if (hit || step[r] == 1 || c == 0) {
+          hi[r] = lo[r];
+        } else {
+          // samples 0 .. c-1 are below the key and sample c (if any) above
+          // it: the key can only lie strictly between samples c-1 and c
+          const OffT nlo = lo[r] + (OffT)(c - 1) * step[r];
+          const OffT nhi = nlo + step[r];
+          lo[r] = nlo + 1;
+          hi[r] = nhi < hi[r] ? nhi : hi[r];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < P; ++r)
+      if (found & (1u << r)) meets |= qm[base + (uint32_t)r * kWave + lane];
   }
   __builtin_amdgcn_wave_barrier();
 }
@@ -741,8 +794,8 @@ __device__ __forceinline__ void bridge_pass(const BatchParams<OffT>& p, unsigned
     while (cols) {
       const int k = __builtin_ctzll(cols);
       cols &= cols - 1;
-      const unsigned long long m = fa & (unsigned long long)shfl64((int64_t)fb, k);
-      const int32_t bk = __shfl(b, k);
+      const unsigned long long m = fa & (unsigned long long)bcast64((int64_t)fb, k);
+      const int32_t bk = bcast(b, k);
       const bool hit = m != 0 && a != bk;
       const uint64_t hm = ballot(hit);
       if (!hm) continue;

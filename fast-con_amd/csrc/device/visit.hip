@@ -60,14 +60,8 @@
 namespace fcd {
 
 template <typename OffT>
-__device__ __forceinline__ OffT shfl_off(OffT v, int src);
-template <>
-__device__ __forceinline__ int32_t shfl_off<int32_t>(int32_t v, int src) {
-  return __shfl(v, src);
-}
-template <>
-__device__ __forceinline__ int64_t shfl_off<int64_t>(int64_t v, int src) {
-  return shfl64(v, src);
+__device__ __forceinline__ OffT bcast_off(OffT v, int src) {
+  return sizeof(OffT) == 4 ? (OffT)bcast((int32_t)v, src) : (OffT)bcast64((int64_t)v, src);
 }
 
 __device__ __forceinline__ bool test_bit(const uint32_t* bm, int32_t v) {
@@ -552,7 +546,7 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
-      const OffT bb = shfl_off<OffT>(b, l), ee = shfl_off<OffT>(e, l);
+      const OffT bb = bcast_off<OffT>(b, l), ee = bcast_off<OffT>(e, l);
       int32_t fp = -1;
       for (OffT j0 = bb; j0 < ee; j0 += kWave) {
         const OffT j = j0 + (OffT)lane;
@@ -564,7 +558,7 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
         }
         const uint64_t hm = ballot(hit);
         if (hm) {
-          fp = __shfl(u, __ffsll((unsigned long long)hm) - 1);
+          fp = bcast(u, __ffsll((unsigned long long)hm) - 1);
           break;
         }
       }

@@ -41,6 +41,16 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool want, u
   return base + lane_rank(m);
 }
 
+// v of lane src, for a src that every lane holds alike (a ballot's bit, a
+// constant): one scalar read of the lane's register instead of an LDS
+// permute. The lane need not be active.
+__device__ __forceinline__ int32_t bcast(int32_t v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ __forceinline__ int64_t bcast64(int64_t v, int src) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), src);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int64_t shfl64(int64_t v, int src) {
   int lo = __shfl((int)(uint32_t)(uint64_t)v, src);
   int hi = __shfl((int)(uint32_t)((uint64_t)v >> 32), src);
