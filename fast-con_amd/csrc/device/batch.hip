@@ -116,8 +116,8 @@ __device__ __forceinline__ void block_reduce_slots(const unsigned long long* val
   const int w = (int)(threadIdx.x >> 6);
 #pragma unroll
   for (int k = 0; k < kRCount; ++k) {
-    const unsigned long long v = k < kRMfS ? (unsigned long long)wave_or_u64(vals[k])
-                                           : (unsigned long long)wave_sum_u64(vals[k]);
+    const unsigned long long v = k < kRMfS ? (unsigned long long)wave_reduce_full_nz_u64<true>(vals[k])
+                                           : (unsigned long long)wave_reduce_full_nz_u64<false>(vals[k]);
     if (lane_id() == 0) sh.red[k][w] = v;
   }
   __syncthreads();
@@ -1431,7 +1431,8 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
 #pragma unroll
     for (int k = 0; k < kRCount; ++k) {
       if (v[k]) row[k] = 0ull;
-      tot.v[k] = k < kRMfS ? (unsigned long long)wave_or_u64(v[k]) : (unsigned long long)wave_sum_u64(v[k]);
+      tot.v[k] = k < kRMfS ? (unsigned long long)wave_reduce_full_nz_u64<true>(v[k])
+                           : (unsigned long long)wave_reduce_full_nz_u64<false>(v[k]);
     }
   }
   __builtin_amdgcn_wave_barrier();

@@ -57,6 +57,35 @@ __device__ __forceinline__ int64_t shfl64(int64_t v, int src) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// Sum or OR of a 64-bit value over a wave whose 64 lanes are all active,
+// with DPP row shifts and row broadcasts (VALU only, no LDS): after the
+// shifts by 1, 2, 4 and 8 lane 15 of each row of 16 holds the row's total,
+// the broadcast of lane 15 adds rows 0 and 2 into rows 1 and 3, and the
+// broadcast of lane 31 adds rows 0-1 into row 3. Lane 63 then holds the
+// wave's total, which every lane reads.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, Ctrl, RowMask, 0xf, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), Ctrl, RowMask, 0xf, true);
+  return ((uint64_t)hi << 32) | lo;
+}
+template <bool Or>
+__device__ __forceinline__ uint64_t wave_reduce_full_u64(uint64_t v) {
+  constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+  v = Or ? v | dpp_u64<kRowShr + 1, 0xf>(v) : v + dpp_u64<kRowShr + 1, 0xf>(v);
+  v = Or ? v | dpp_u64<kRowShr + 2, 0xf>(v) : v + dpp_u64<kRowShr + 2, 0xf>(v);
+  v = Or ? v | dpp_u64<kRowShr + 4, 0xf>(v) : v + dpp_u64<kRowShr + 4, 0xf>(v);
+  v = Or ? v | dpp_u64<kRowShr + 8, 0xf>(v) : v + dpp_u64<kRowShr + 8, 0xf>(v);
+  v = Or ? v | dpp_u64<kRowBcast15, 0xa>(v) : v + dpp_u64<kRowBcast15, 0xa>(v);
+  v = Or ? v | dpp_u64<kRowBcast31, 0xc>(v) : v + dpp_u64<kRowBcast31, 0xc>(v);
+  return (uint64_t)bcast64((int64_t)v, 63);
+}
+// The same, 0 without any work when every lane holds 0.
+template <bool Or>
+__device__ __forceinline__ uint64_t wave_reduce_full_nz_u64(uint64_t v) {
+  return ballot(v != 0) ? wave_reduce_full_u64<Or>(v) : 0ull;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) {
     uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o);
