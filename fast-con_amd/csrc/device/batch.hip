@@ -116,8 +116,8 @@ __device__ __forceinline__ void block_reduce_slots(const unsigned long long* val
   const int w = (int)(threadIdx.x >> 6);
 #pragma unroll
   for (int k = 0; k < kRCount; ++k) {
-    const unsigned long long v = k < kRMfS ? (unsigned long long)wave_reduce_full_nz_u64<true>(vals[k])
-                                           : (unsigned long long)wave_reduce_full_nz_u64<false>(vals[k]);
+    const unsigned long long v = k < kRMfS ? (unsigned long long)wave_reduce_full_nz_u64<kWaveOr>(vals[k])
+                                           : (unsigned long long)wave_reduce_full_nz_u64<kWaveSum>(vals[k]);
     if (lane_id() == 0) sh.red[k][w] = v;
   }
   __syncthreads();
@@ -170,7 +170,7 @@ __device__ __forceinline__ void appends_block_flush(const BatchParams<OffT>& p, 
   unsigned long long he[2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
-    he[x] = wave_sum_u64((uint32_t)lane < ap.h[x].cnt ? (unsigned long long)ap.h[x].d[lane] : 0ull);
+    he[x] = wave_reduce_full_nz_u64<kWaveSum>((uint32_t)lane < ap.h[x].cnt ? (unsigned long long)ap.h[x].d[lane] : 0ull);
   if (lane == 0) {
     sh.cnt[0][w] = ap.s[0].cnt;
     sh.cnt[1][w] = ap.s[1].cnt;
@@ -1431,8 +1431,8 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
 #pragma unroll
     for (int k = 0; k < kRCount; ++k) {
       if (v[k]) row[k] = 0ull;
-      tot.v[k] = k < kRMfS ? (unsigned long long)wave_reduce_full_nz_u64<true>(v[k])
-                           : (unsigned long long)wave_reduce_full_nz_u64<false>(v[k]);
+      tot.v[k] = k < kRMfS ? (unsigned long long)wave_reduce_full_nz_u64<kWaveOr>(v[k])
+                           : (unsigned long long)wave_reduce_full_nz_u64<kWaveSum>(v[k]);
     }
   }
   __builtin_amdgcn_wave_barrier();

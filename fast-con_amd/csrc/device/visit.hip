@@ -137,16 +137,6 @@ __device__ __forceinline__ void push_next(Appender<OffT>& a, bool claimed, int32
   }
 }
 
-__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), o);
-    const unsigned long long w = ((unsigned long long)hi << 32) | lo;
-    v = w < v ? w : v;
-  }
-  return v;
-}
-
 // Block-collective publication of both colours' staged entries (all threads,
 // uniform control flow, at the end of the level). One barrier publishes every
 // wave's counts; then four waves each take one list with one atomic, at the
@@ -170,9 +160,10 @@ __device__ __forceinline__ void block_flush(Appender<OffT>& a, int ncol, BlockSh
   const int lane = lane_id();
   const int own = a.c;  // this wave staged entries of its own colour only
   {
-    const unsigned long long mf = (unsigned long long)wave_sum_u64(a.mf);
-    const unsigned long long best = wave_min_u64(a.best);
-    const unsigned long long hd = wave_sum_u64((uint32_t)lane < a.h.cnt ? (unsigned long long)a.h.d[lane] : 0ull);
+    const unsigned long long mf = (unsigned long long)wave_reduce_full_nz_u64<kWaveSum>(a.mf);
+    const unsigned long long best = wave_reduce_full_nz_u64<kWaveMin>(a.best);
+    const unsigned long long hd =
+        wave_reduce_full_nz_u64<kWaveSum>((uint32_t)lane < a.h.cnt ? (unsigned long long)a.h.d[lane] : 0ull);
     if (lane == 0) {
       sh.cnt[own][w] = a.cnt;
       sh.mf[own][w] = mf;

@@ -57,33 +57,48 @@ __device__ __forceinline__ int64_t shfl64(int64_t v, int src) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Sum or OR of a 64-bit value over a wave whose 64 lanes are all active,
-// with DPP row shifts and row broadcasts (VALU only, no LDS): after the
-// shifts by 1, 2, 4 and 8 lane 15 of each row of 16 holds the row's total,
-// the broadcast of lane 15 adds rows 0 and 2 into rows 1 and 3, and the
-// broadcast of lane 31 adds rows 0-1 into row 3. Lane 63 then holds the
-// wave's total, which every lane reads.
-template <int Ctrl, int RowMask>
+// Sum, OR or minimum of a 64-bit value over a wave whose 64 lanes are all
+// active, with DPP row shifts and row broadcasts (VALU only, no LDS): after
+// the shifts by 1, 2, 4 and 8 lane 15 of each row of 16 holds the row's
+// total, the broadcast of lane 15 folds rows 0 and 2 into rows 1 and 3, and
+// the broadcast of lane 31 folds rows 0-1 into row 3. Lane 63 then holds the
+// wave's total, which every lane reads. Lanes that a step has no source for
+// take the operation's identity.
+enum WaveOp : int { kWaveSum, kWaveOr, kWaveMin };
+
+template <int Op>
+__device__ __forceinline__ uint64_t wave_apply(uint64_t a, uint64_t b) {
+  return Op == kWaveSum ? a + b : Op == kWaveOr ? (a | b) : (b < a ? b : a);
+}
+
+template <int Op, int Ctrl, int RowMask>
 __device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, Ctrl, RowMask, 0xf, true);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), Ctrl, RowMask, 0xf, true);
+  // the identity: 0 written for lanes without a source (bound_ctrl), or the
+  // all-ones old value kept for them
+  constexpr bool kOnes = Op == kWaveMin;
+  const int old = kOnes ? -1 : 0;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(old, (int)(uint32_t)v, Ctrl, RowMask, 0xf, !kOnes);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(old, (int)(uint32_t)(v >> 32), Ctrl, RowMask, 0xf, !kOnes);
   return ((uint64_t)hi << 32) | lo;
 }
-template <bool Or>
+
+template <int Op>
 __device__ __forceinline__ uint64_t wave_reduce_full_u64(uint64_t v) {
   constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
-  v = Or ? v | dpp_u64<kRowShr + 1, 0xf>(v) : v + dpp_u64<kRowShr + 1, 0xf>(v);
-  v = Or ? v | dpp_u64<kRowShr + 2, 0xf>(v) : v + dpp_u64<kRowShr + 2, 0xf>(v);
-  v = Or ? v | dpp_u64<kRowShr + 4, 0xf>(v) : v + dpp_u64<kRowShr + 4, 0xf>(v);
-  v = Or ? v | dpp_u64<kRowShr + 8, 0xf>(v) : v + dpp_u64<kRowShr + 8, 0xf>(v);
-  v = Or ? v | dpp_u64<kRowBcast15, 0xa>(v) : v + dpp_u64<kRowBcast15, 0xa>(v);
-  v = Or ? v | dpp_u64<kRowBcast31, 0xc>(v) : v + dpp_u64<kRowBcast31, 0xc>(v);
+  v = wave_apply<Op>(v, dpp_u64<Op, kRowShr + 1, 0xf>(v));
+  v = wave_apply<Op>(v, dpp_u64<Op, kRowShr + 2, 0xf>(v));
+  v = wave_apply<Op>(v, dpp_u64<Op, kRowShr + 4, 0xf>(v));
+  v = wave_apply<Op>(v, dpp_u64<Op, kRowShr + 8, 0xf>(v));
+  v = wave_apply<Op>(v, dpp_u64<Op, kRowBcast15, 0xa>(v));
+  v = wave_apply<Op>(v, dpp_u64<Op, kRowBcast31, 0xc>(v));
   return (uint64_t)bcast64((int64_t)v, 63);
 }
-// The same, 0 without any work when every lane holds 0.
-template <bool Or>
+
+// The same, without any work when every lane holds the identity.
+template <int Op>
 __device__ __forceinline__ uint64_t wave_reduce_full_nz_u64(uint64_t v) {
-  return ballot(v != 0) ? wave_reduce_full_u64<Or>(v) : 0ull;
+  constexpr uint64_t id = Op == kWaveMin ? ~0ull : 0ull;
+  return ballot(v != id) ? wave_reduce_full_u64<Op>(v) : id;
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
