@@ -857,11 +857,12 @@ __device__ __forceinline__ void batch_features(BatchCtl* ctl, const uint64_t* mf
   f[11] = (float)((double)ctl->vis_pairs[1] / pairs);
 }
 
+// The smallest power of two in [4, 64] that is at least the mean degree
+// ceil(mf / n): vw < ceil(mf / n) exactly when vw * n < mf, so no division.
 __device__ __forceinline__ int bdegree_width(uint64_t n, uint64_t mf) {
   if (n == 0) return 4;
-  const uint64_t avg = (mf + n - 1) / n;
   int vw = 4;
-  while (vw < 64 && (uint64_t)vw < avg) vw <<= 1;
+  while (vw < 64 && (uint64_t)vw * n < mf) vw <<= 1;
   return vw;
 }
 
@@ -877,6 +878,19 @@ __device__ __forceinline__ void batch_apply_action(BatchDecision& d, int a) {
 // (default; FASTCON_BOTH_MAX_EDGES overrides it).
 constexpr uint64_t kBothMaxEdges = 1u << 20;
 
+// floor(x / d) for x < 2^53 and 1 <= d <= 64 without a 64-bit integer
+// division (a long instruction sequence on this chip): the double quotient is
+// exact to within 2^-53 relative, so its truncation is off by at most one,
+// which one comparison corrects.
+__device__ __forceinline__ uint64_t div_small(uint64_t x, uint32_t d) {
+  uint64_t q = (uint64_t)((double)x / (double)d);
+  if (q * d > x)
+    --q;
+  else if ((q + 1) * d <= x)
+    ++q;
+  return q;
+}
+
 // Degree sum of side x's current frontier restricted to the live queries,
 // estimated from the fraction of the queries the lists were built for that
 // are still live (exact right after a filter).
@@ -885,7 +899,7 @@ __device__ __forceinline__ uint64_t live_mf(const BatchCtl* ctl, int x) {
   const int built = __popcll(ctl->lact[x]);
   const int live = __popcll(ctl->lact[x] & ctl->act);
   if (built == 0 || live == built) return mf;
-  return mf * (uint64_t)live / (uint64_t)built;  // mf < 2^40 and live <= 64: no overflow
+  return div_small(mf * (uint64_t)live, (uint32_t)built);  // mf < 2^40 and live <= 64: no overflow
 }
 
 template <typename OffT>
@@ -992,10 +1006,19 @@ __device__ __forceinline__ int batch_next(BatchCtl* ctl, bool allow_filter) {
 struct FinPre {
   uint32_t ln[2];
   unsigned long long lh[2], lm[2], act0;
+  uint32_t next_wave;  // cleanup: the wave claimed from the shared counter
+};
+
+// Distances a finalisation resolves: base + 1 for the queries in odd, base +
+// 2 for those in even, -1 (not connected) for those in dead. The whole
+// finalising wave writes them, one lane per query.
+struct FinDist {
+  unsigned long long odd, even, dead;
+  int base;
 };
 
 __device__ __forceinline__ FinPre fin_prefetch(const BatchCtl* ctl, int phase, const int* nxt) {
-  FinPre f{{0u, 0u}, {0ull, 0ull}, {0ull, 0ull}, 0ull};
+  FinPre f{{0u, 0u}, {0ull, 0ull}, {0ull, 0ull}, 0ull, 0u};
   if (phase == kBInit) {
     f.act0 = ctl->act;
 #pragma unroll
@@ -1017,8 +1040,8 @@ __device__ __forceinline__ FinPre fin_prefetch(const BatchCtl* ctl, int phase, c
 // ctl: the finalising wave's copy of the control block's state (LDS).
 template <typename OffT>
 __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, BatchCtl* ctl, int phase,
-                                              const BatchDecision& d, const RedTot& r, const FinPre& f) {
-  const int wave0 = ctl->wave * 64;
+                                              const BatchDecision& d, const RedTot& r, const FinPre& f,
+                                              FinDist& fd) {
   const uint32_t ln[2] = {f.ln[0], f.ln[1]};
   const unsigned long long lh[2] = {f.lh[0], f.lh[1]}, lm[2] = {f.lm[0], f.lm[1]}, act0 = f.act0;
   ctl->active_vertices += (int64_t)r.v[kRActV];
@@ -1054,12 +1077,7 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, BatchC
       if (d.sides & kSideS) alive &= r.v[kRAnyS];
       if (d.sides & kSideT) alive &= r.v[kRAnyT];
       const unsigned long long dead = act & ~alive & ~odd & ~even;
-      const int base = ctl->lvl[0] + ctl->lvl[1];
-      for (unsigned long long m = odd | even | dead; m; m &= m - 1) {
-        const int q = __builtin_ctzll(m);
-        const unsigned long long b = 1ull << q;
-        p.dist[wave0 + q] = (odd & b) ? base + 1 : (even & b) ? base + 2 : -1;
-      }
+      fd = FinDist{odd, even, dead, ctl->lvl[0] + ctl->lvl[1]};
       ctl->act = act & ~(odd | even | dead);
       ctl->steps++;
       ctl->levels_total++;
@@ -1091,8 +1109,7 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, BatchC
     case kBBridge: {
       const unsigned long long act = ctl->act;
       const unsigned long long hit = r.v[kROdd] & act;
-      const int base = ctl->lvl[0] + ctl->lvl[1];
-      for (unsigned long long m = hit; m; m &= m - 1) p.dist[wave0 + __builtin_ctzll(m)] = base + 1;
+      fd = FinDist{hit, 0ull, 0ull, ctl->lvl[0] + ctl->lvl[1]};
       ctl->act = act & ~hit;
       ctl->bridges++;
       ctl->bridged += __popcll(hit);
@@ -1141,7 +1158,7 @@ __device__ __forceinline__ int batch_finalize(const BatchParams<OffT>& p, BatchC
       // the next wave: claimed from the counter the concurrent machines
       // share (so that a machine whose waves ran fast takes more of them),
       // or this machine's next one
-      ctl->wave = p.wave_ticket ? (int32_t)atomicAdd(p.wave_ticket, 1u) : ctl->wave + 1;
+      ctl->wave = p.wave_ticket ? (int32_t)f.next_wave : ctl->wave + 1;
       ctl->t_wave_end = (int64_t)wall_clock64();
       ctl->phase = ctl->wave < ctl->nwaves && ctl->wave < ctl->stop_wave ? kBInit : kBIdle;
       return kFinDone;
@@ -1415,6 +1432,10 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   // The state words and the launch's reduction slots are loaded together,
   // in one round trip.
   __shared__ __align__(16) unsigned long long s_ctl[kCtlHotWords];
+  // A cleanup claims the machine's next wave now, so that the counter's
+  // round trip overlaps the state loads.
+  uint32_t next_wave = 0;
+  if (phase == kBCleanup && p.wave_ticket && lane_id() == 0) next_wave = atomicAdd(p.wave_ticket, 1u);
   RedTot tot;
   {
     const int lane = lane_id();
@@ -1439,7 +1460,10 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
   BatchCtl* lc = reinterpret_cast<BatchCtl*>(s_ctl);
   int need = kFinDone;
   {
-    const FinPre pre = fin_prefetch(lc, phase, nxt);
+    FinPre pre = fin_prefetch(lc, phase, nxt);
+    pre.next_wave = next_wave;
+    const int wave0 = lc->wave * 64;
+    FinDist fd{0ull, 0ull, 0ull, 0};
     if (threadIdx.x == 0) {
       const int64_t ph = lc->phases;
       if (ph < kPhaseLog) {
@@ -1451,7 +1475,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
       if (phase == kBDecide) {
         need = lc->pause_filter ? kFinDecideFilter : kFinDecide;
       } else {
-        need = batch_finalize(p, lc, phase, d, tot, pre);
+        need = batch_finalize(p, lc, phase, d, tot, pre, fd);
         if (need != kFinDone && bridge_due(p, lc)) {
           lc->bridge_lv = ((lc->lvl[0] + 1) << 16) | (lc->lvl[1] + 1);
           lc->phase = kBBridge;
@@ -1459,6 +1483,14 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(BatchParams<OffT> p) {
         }
       }
     }
+    // the resolved queries' distances, one lane per query
+    const unsigned long long odd = (unsigned long long)bcast64((int64_t)fd.odd, 0);
+    const unsigned long long even = (unsigned long long)bcast64((int64_t)fd.even, 0);
+    const unsigned long long dead = (unsigned long long)bcast64((int64_t)fd.dead, 0);
+    const int base = bcast(fd.base, 0);
+    const int q = lane_id();
+    const unsigned long long b = 1ull << q;
+    if ((odd | even | dead) & b) p.dist[wave0 + q] = (odd & b) ? base + 1 : (even & b) ? base + 2 : -1;
   }
   need = __shfl(need, 0);
   if (need != kFinDone) {
