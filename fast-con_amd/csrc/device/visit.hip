@@ -464,7 +464,8 @@ __device__ __forceinline__ void td_root(Appender<OffT>& a, int32_t* parent, int3
 }
 
 // Candidates of degree <= kBuLane scan their own list; longer lists are
-// scanned by the whole wave, 64 edges at a time.
+// scanned by the whole wave, 64 edges at a time, after the lane's own first
+// round of eight.
 constexpr int kBuLane = 128;
 
 // Per-chunk loads of a bottom-up step (64 consecutive vertices of one wave):
@@ -517,14 +518,18 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
     const OffT b = cur.b, e = cur.e;
     const bool small = cand && (e - b) <= (OffT)kBuLane;
     int32_t par = -1;
-    if (small) {
+    // A long list's first eight entries are scanned in the lane's own first
+    // round too: they hold the parent of nearly every candidate (lists are in
+    // id order, which puts the hubs first), so few long lists need the wave.
+    const OffT lim = small ? e : (b + 8 < e ? b + 8 : e);
+    if (cand) {
       // eight independent column loads per round, then the eight bitmap
       // words together, so a round costs two round trips whichever column
       // holds the parent; most candidates find one in the first round
-      for (OffT j = b; j < e && par < 0; j += 8) {
+      for (OffT j = b; j < lim && par < 0; j += 8) {
         int32_t u[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) u[k] = j + k < e ? p.col[j + k] : -1;
+        for (int k = 0; k < 8; ++k) u[k] = j + k < lim ? p.col[j + k] : -1;
         uint32_t fw[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) fw[k] = u[k] >= 0 ? cfbm[u[k] >> 5] : 0u;
@@ -533,11 +538,11 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
           if (u[k] >= 0 && ((fw[k] >> (u[k] & 31)) & 1u)) par = u[k];
       }
     }
-    uint64_t big = ballot(cand && !small);
+    uint64_t big = ballot(cand && !small && par < 0);
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
-      const OffT bb = bcast_off<OffT>(b, l), ee = bcast_off<OffT>(e, l);
+      const OffT bb = bcast_off<OffT>(lim, l), ee = bcast_off<OffT>(e, l);
       int32_t fp = -1;
       for (OffT j0 = bb; j0 < ee; j0 += kWave) {
         const OffT j = j0 + (OffT)lane;
