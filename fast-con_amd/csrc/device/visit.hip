@@ -505,82 +505,103 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
   // parent: their chunks are skipped (43% of the vertices of R-MAT s22).
   const uint32_t nw64 = ((uint32_t)p.v_active + 63u) >> 6;
   const int32_t mine = lvl + 1;
-  uint32_t w = (uint32_t)gwave;
-  if (w >= nw64) return;
-  BuChunk<OffT> cur = bu_load(p, w);
-  for (;;) {
-    const uint32_t wn = w + (uint32_t)nwaves;
-    const bool more = wn < nw64;
-    BuChunk<OffT> nx;
-    if (more) nx = bu_load(p, wn);
-    const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
-    const bool cand = cur.unvis;
-    const OffT b = cur.b, e = cur.e;
-    const bool small = cand && (e - b) <= (OffT)kBuLane;
-    int32_t par = -1;
-    // A long list's first eight entries are scanned in the lane's own first
-    // round too: they hold the parent of nearly every candidate (lists are in
-    // id order, which puts the hubs first), so few long lists need the wave.
-    const OffT lim = small ? e : (b + 8 < e ? b + 8 : e);
-    if (cand) {
-      // eight independent column loads per round, then the eight bitmap
-      // words together, so a round costs two round trips whichever column
-      // holds the parent; most candidates find one in the first round
-      for (OffT j = b; j < lim && par < 0; j += 8) {
-        int32_t u[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) u[k] = j + k < lim ? p.col[j + k] : -1;
-        uint32_t fw[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) fw[k] = u[k] >= 0 ? cfbm[u[k] >> 5] : 0u;
-#pragma unroll
-        for (int k = 7; k >= 0; --k)
-          if (u[k] >= 0 && ((fw[k] >> (u[k] & 31)) & 1u)) par = u[k];
+  // The wave's chunks are w = gwave + i * nwaves. Lane j first reads the
+  // visited words of the j-th of the next 64 of them, so that chunks without
+  // an unvisited vertex (most of them after the big bottom-up level) cost
+  // nothing beyond that one round trip; the others are processed in order,
+  // each loading the next one's visited bits and offsets while it works.
+  for (uint64_t g0 = (uint64_t)gwave; g0 < nw64; g0 += (uint64_t)kWave * (uint64_t)nwaves) {
+    const uint64_t cj = g0 + (uint64_t)lane * (uint64_t)nwaves;
+    bool work = false;
+    if (cj < nw64) {
+      const uint64_t vis = ((uint64_t)p.visited[2 * cj + 1] << 32) | p.visited[2 * cj];
+      const int64_t left = (int64_t)p.V - (int64_t)cj * 64;  // vertices of the chunk below V
+      const uint64_t valid = left >= 64 ? ~0ull : (1ull << left) - 1ull;
+      work = (~vis & valid) != 0;
+    }
+    uint64_t todo = ballot(work);
+    if (!todo) continue;
+    uint32_t w = (uint32_t)(g0 + (uint64_t)__builtin_ctzll(todo) * (uint64_t)nwaves);
+    todo &= todo - 1;
+    BuChunk<OffT> cur = bu_load(p, w);
+    for (;;) {
+      const bool more = todo != 0;
+      uint32_t wn = 0;
+      BuChunk<OffT> nx;
+      if (more) {
+        wn = (uint32_t)(g0 + (uint64_t)__builtin_ctzll(todo) * (uint64_t)nwaves);
+        todo &= todo - 1;
+        nx = bu_load(p, wn);
       }
-    }
-    uint64_t big = ballot(cand && !small && par < 0);
-    while (big) {
-      const int l = __ffsll((unsigned long long)big) - 1;
-      big &= big - 1;
-      const OffT bb = bcast_off<OffT>(lim, l), ee = bcast_off<OffT>(e, l);
-      int32_t fp = -1;
-      for (OffT j0 = bb; j0 < ee; j0 += kWave) {
-        const OffT j = j0 + (OffT)lane;
-        bool hit = false;
-        int32_t u = 0;
-        if (j < ee) {
-          u = p.col[j];
-          hit = test_bit(cfbm, u);
-        }
-        const uint64_t hm = ballot(hit);
-        if (hm) {
-          fp = bcast(u, __ffsll((unsigned long long)hm) - 1);
-          break;
+      const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
+      const bool cand = cur.unvis;
+      const OffT b = cur.b, e = cur.e;
+      const bool small = cand && (e - b) <= (OffT)kBuLane;
+      int32_t par = -1;
+      // A long list's first eight entries are scanned in the lane's own first
+      // round too: they hold the parent of nearly every candidate (lists are in
+      // id order, which puts the hubs first), so few long lists need the wave.
+      const OffT lim = small ? e : (b + 8 < e ? b + 8 : e);
+      if (cand) {
+        // eight independent column loads per round, then the eight bitmap
+        // words together, so a round costs two round trips whichever column
+        // holds the parent; most candidates find one in the first round
+        for (OffT j = b; j < lim && par < 0; j += 8) {
+          int32_t u[8];
+  #pragma unroll
+          for (int k = 0; k < 8; ++k) u[k] = j + k < lim ? p.col[j + k] : -1;
+          uint32_t fw[8];
+  #pragma unroll
+          for (int k = 0; k < 8; ++k) fw[k] = u[k] >= 0 ? cfbm[u[k] >> 5] : 0u;
+  #pragma unroll
+          for (int k = 7; k >= 0; --k)
+            if (u[k] >= 0 && ((fw[k] >> (u[k] & 31)) & 1u)) par = u[k];
         }
       }
-      if (lane == l) par = fp;
+      uint64_t big = ballot(cand && !small && par < 0);
+      while (big) {
+        const int l = __ffsll((unsigned long long)big) - 1;
+        big &= big - 1;
+        const OffT bb = bcast_off<OffT>(lim, l), ee = bcast_off<OffT>(e, l);
+        int32_t fp = -1;
+        for (OffT j0 = bb; j0 < ee; j0 += kWave) {
+          const OffT j = j0 + (OffT)lane;
+          bool hit = false;
+          int32_t u = 0;
+          if (j < ee) {
+            u = p.col[j];
+            hit = test_bit(cfbm, u);
+          }
+          const uint64_t hm = ballot(hit);
+          if (hm) {
+            fp = bcast(u, __ffsll((unsigned long long)hm) - 1);
+            break;
+          }
+        }
+        if (lane == l) par = fp;
+      }
+      const bool claimed = cand && par >= 0;
+      if (claimed) {
+        p.state[v] = mine;
+        if (parent) parent[v] = par;
+      }
+      // Bitmap words of these 64 vertices belong to this wave during a
+      // bottom-up level, so they are updated without atomics (and the
+      // prefetched words of the wave's next chunk are not changed by others).
+      const uint64_t cm = ballot(claimed);
+      if (lane == 0 && (uint32_t)cm) {
+        p.visited[2u * w] |= (uint32_t)cm;
+        nfbm[2u * w] = (uint32_t)cm;
+      }
+      if (lane == 32 && (uint32_t)(cm >> 32)) {
+        p.visited[2u * w + 1u] |= (uint32_t)(cm >> 32);
+        nfbm[2u * w + 1u] = (uint32_t)(cm >> 32);
+      }
+      push_next(a, claimed, v, claimed ? (int64_t)(e - b) : 0, false);
+      if (!more) break;
+      cur = nx;
+      w = wn;
     }
-    const bool claimed = cand && par >= 0;
-    if (claimed) {
-      p.state[v] = mine;
-      if (parent) parent[v] = par;
-    }
-    // Bitmap words of these 64 vertices belong to this wave during a
-    // bottom-up level, so they are updated without atomics (and the
-    // prefetched words of the wave's next chunk are not changed by others).
-    const uint64_t cm = ballot(claimed);
-    if (lane == 0 && (uint32_t)cm) {
-      p.visited[2u * w] |= (uint32_t)cm;
-      nfbm[2u * w] = (uint32_t)cm;
-    }
-    if (lane == 32 && (uint32_t)(cm >> 32)) {
-      p.visited[2u * w + 1u] |= (uint32_t)(cm >> 32);
-      nfbm[2u * w + 1u] = (uint32_t)(cm >> 32);
-    }
-    push_next(a, claimed, v, claimed ? (int64_t)(e - b) : 0, false);
-    if (!more) break;
-    cur = nx;
-    w = wn;
   }
 }
 
@@ -598,11 +619,12 @@ __device__ __forceinline__ void td_dispatch(Appender<OffT>& a, int32_t* parent, 
 
 // Width for a top-down level: the smallest power of two >= the average degree
 // of the normal-queue entries, clamped to [4, 64].
+// Every block decides this at launch start: vw < ceil(mf / n) exactly when
+// vw * n < mf, so no 64-bit division.
 __device__ __forceinline__ int degree_width(uint32_t n, uint64_t mf_normal) {
   if (n == 0) return 4;
-  const uint64_t avg = (mf_normal + n - 1) / n;
   int vw = 4;
-  while (vw < 64 && (uint64_t)vw < avg) vw <<= 1;
+  while (vw < 64 && (uint64_t)vw * n < mf_normal) vw <<= 1;
   return vw;
 }
 
