@@ -416,20 +416,25 @@ struct GroupDeal {
   __device__ __forceinline__ int64_t operator()(int k) const { return (jb + k) * (int64_t)nwaves + gwave; }
 };
 
-// Number of groups owned by wave gwave.
+// Number of groups owned by wave gwave (groups = ceil(V / 32) < 2^27, so
+// a 32-bit division does).
 __device__ __forceinline__ int64_t owned_groups(int64_t groups, int gwave, int nwaves) {
-  return groups > gwave ? (groups - gwave + nwaves - 1) / nwaves : 0;
+  return groups > gwave ? (int64_t)(((uint32_t)groups - (uint32_t)gwave + (uint32_t)nwaves - 1u) / (uint32_t)nwaves)
+                        : 0;
 }
 
 // Mark dirty vertices (visited words became non-zero this wave) in the dirty
 // bitmap, for a pass whose lanes 0..31 / 32..63 hold the 32 vertices of
-// groups g0 / g1 (each group owned by this wave in this launch).
+// groups g0 / g1 (each group owned by this wave in this launch). dbw is the
+// group's dirty word as loaded with the pass's records, so that the update
+// waits for no load of its own.
 template <typename OffT>
-__device__ __forceinline__ void mark_dirty(const BatchParams<OffT>& p, bool dirty, int64_t g0, int64_t g1) {
+__device__ __forceinline__ void mark_dirty(const BatchParams<OffT>& p, bool dirty, int64_t g0, int64_t g1,
+                                           uint32_t dbw) {
   const uint64_t dm = ballot(dirty);
   const int lane = lane_id();
-  if (lane == 0 && (uint32_t)dm) p.db[g0] |= (uint32_t)dm;
-  if (lane == 32 && (dm >> 32) && g1 >= 0) p.db[g1] |= (uint32_t)(dm >> 32);
+  if (lane == 0 && (uint32_t)dm) p.db[g0] = dbw | (uint32_t)dm;
+  if (lane == 32 && (dm >> 32) && g1 >= 0) p.db[g1] = dbw | (uint32_t)(dm >> 32);
 }
 
 // Commit: every vertex of a touched group folds its N words into its visited
@@ -446,6 +451,7 @@ struct CommitLoad {
   ulonglong2 a, b;  // {visS, NS}, {visT, NT}
   OffT o0, o1;
   int32_t u;
+  uint32_t dbw;     // the dirty-bitmap word of the lane's group
   bool valid;
 };
 
@@ -460,6 +466,7 @@ __device__ __forceinline__ CommitLoad<OffT> commit_load(const BatchParams<OffT>&
   c.valid = mine >= 0 && u < p.V;
   c.a = c.b = make_ulonglong2(0ull, 0ull);
   c.o0 = c.o1 = 0;
+  c.dbw = mine >= 0 ? p.db[gd(mine)] : 0u;
   if (c.valid) {
     c.a = *reinterpret_cast<const ulonglong2*>(rec_of(p, 0, u));
     c.b = *reinterpret_cast<const ulonglong2*>(rec_of(p, 1, u));
@@ -525,7 +532,7 @@ __device__ __forceinline__ void commit_pass(const BatchParams<OffT>& p, Appends&
       }
       push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
       push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
-      mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), gd(k0), k1 >= 0 ? gd(k1) : (int64_t)-1);
+      mark_dirty(p, ((vS | vT) == 0) && ((nS | nT) != 0), gd(k0), k1 >= 0 ? gd(k1) : (int64_t)-1, cur.dbw);
       if (!more) break;
       cur = nx;
       k0 = n0;
@@ -553,6 +560,8 @@ __device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& a
     const int32_t u = (int32_t)(base + lane);
     unsigned long long accS = 0, accT = 0, needS = 0, needT = 0, vS = 0, vT = 0;
     OffT b = 0, e = 0;
+    // this lane's group's dirty word (lanes 0..31: base / 32, 32..63: the next)
+    const uint32_t dbw = base + (lane & 32) < p.V ? p.db[(base >> 5) + (lane >> 5)] : 0u;
     if (u < p.V) {
       vS = rec_of(p, 0, u)[0];
       vT = rec_of(p, 1, u)[0];
@@ -601,7 +610,7 @@ __device__ __forceinline__ void bottom_up(const BatchParams<OffT>& p, Appends& a
     if (DS) push_frontier(p, ap, 0, nxt[0], nS != 0, u, deg);
     if (DT) push_frontier(p, ap, 1, nxt[1], nT != 0, u, deg);
     // vertices base .. base+63 are this wave's alone in this launch
-    mark_dirty(p, (nS | nT) != 0 && (vS | vT) == 0, base >> 5, (base >> 5) + 1);
+    mark_dirty(p, (nS | nT) != 0 && (vS | vT) == 0, base >> 5, (base >> 5) + 1, dbw);
   }
 }
 
@@ -771,8 +780,11 @@ __device__ __forceinline__ void bridge_pass(const BatchParams<OffT>& p, unsigned
   const uint64_t tiles = (uint64_t)((nS + kWave - 1) / kWave) * ct;
   uint32_t cnt = 0, tested = 0;
   for (uint64_t tile = (uint64_t)gwave; tile < tiles && tested < kBridgeMaxTests; tile += (uint64_t)nwaves) {
-    const uint32_t i = (uint32_t)(tile / ct) * kWave + (uint32_t)lane;
-    const uint32_t j = (uint32_t)(tile % ct) * kWave + (uint32_t)lane;
+    // tiles < 2^32: a bridge step runs only when the lists' product is at
+    // most kBridgeMaxPairs
+    const uint32_t ti = (uint32_t)tile / ct;
+    const uint32_t i = ti * kWave + (uint32_t)lane;
+    const uint32_t j = ((uint32_t)tile - ti * ct) * kWave + (uint32_t)lane;
     int32_t a = -1, b = -1;
     unsigned long long fa = 0, fb = 0;
     if (i < nS) {
