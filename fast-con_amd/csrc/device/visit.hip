@@ -476,6 +476,7 @@ template <typename OffT>
 struct BuChunk {
   bool unvis;
   OffT b, e;
+  uint32_t vword;  // the lane's visited word (lanes 0..31: 2w, 32..63: 2w + 1)
 };
 
 template <typename OffT>
@@ -484,7 +485,8 @@ __device__ __forceinline__ BuChunk<OffT> bu_load(const VisitParams<OffT>& p, uin
   const int32_t v = (int32_t)(w * 64u + (uint32_t)lane);
   const uint32_t wi = 2u * w + ((uint32_t)lane >> 5);
   BuChunk<OffT> c;
-  c.unvis = v < p.V && !((p.visited[wi] >> (lane & 31)) & 1u);
+  c.vword = p.visited[wi];
+  c.unvis = v < p.V && !((c.vword >> (lane & 31)) & 1u);
   c.b = c.e = 0;
   if (c.unvis) {
     c.b = p.off[v];
@@ -589,12 +591,13 @@ __device__ __forceinline__ void bu_step(Appender<OffT>& a, int32_t* parent, int 
       // bottom-up level, so they are updated without atomics (and the
       // prefetched words of the wave's next chunk are not changed by others).
       const uint64_t cm = ballot(claimed);
+      // (the visited words came with the chunk's loads: no load waits here)
       if (lane == 0 && (uint32_t)cm) {
-        p.visited[2u * w] |= (uint32_t)cm;
+        p.visited[2u * w] = cur.vword | (uint32_t)cm;
         nfbm[2u * w] = (uint32_t)cm;
       }
       if (lane == 32 && (uint32_t)(cm >> 32)) {
-        p.visited[2u * w + 1u] |= (uint32_t)(cm >> 32);
+        p.visited[2u * w + 1u] = cur.vword | (uint32_t)(cm >> 32);
         nfbm[2u * w + 1u] = (uint32_t)(cm >> 32);
       }
       push_next(a, claimed, v, claimed ? (int64_t)(e - b) : 0, false);
