@@ -110,15 +110,6 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
-__device__ __forceinline__ uint64_t wave_or_u64(uint64_t v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o);
-    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), o);
-    v |= ((uint64_t)hi << 32) | lo;
-  }
-  return v;
-}
-
 template <typename T>
 __device__ __forceinline__ T ld_agent(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
