@@ -893,7 +893,8 @@ constexpr uint64_t kBothMaxEdges = 1u << 20;
 // frontier has more than this many times the other's edges. Measured: s22
 // (1,024 pairs) 2.06 ms at 4, 2.08 at 16 and 2.08 with no bound; s24
 // (16,384 pairs, 30% with an endpoint outside the giant component) 89.6,
-// 107.6 and 134.8 ms. Never expanding both: s22 2.31 ms, s24 96.5 ms.
+// 107.6 and 134.8 ms. Never expanding both: s22 2.31 ms, s24 96.5 ms. A
+// bound of 2 gave s24 88.0 ms but s22 2.15 ms; 8 gave 96.1 and 2.04 ms.
 constexpr uint64_t kBothMaxRatio = 4;
 
 // floor(x / d) for x < 2^53 and 1 <= d <= 64 without a 64-bit integer
